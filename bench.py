@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the benchNTT hot path on one MI355X.
+
+Default workload (the BLAS configuration): axpy mod q_B = 2^124 - 59 over
+vectors of 2^24 seeded 128-bit residues (256 MiB each), c = a + s*b.  One step
+is one axpy call over the whole vectors with inputs resident in HBM.  `value`
+is GB/s of algorithmic traffic (48 B per element: two operands in, one out).
+
+The same JSON line also carries vvadd/vvsub/vvmul at the same size, the NTT
+rows (plain N=1024 transform per launch, uniform batch 8192 x N=1024), the
+roofline of the axpy kernel and a CPU baseline: the reference's own axpy
+(oracle/_ref, AVX-512 / AVX2 / scalar as the host supports) timed on this
+host's cores over the same vectors.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_ELEM = 48
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def time_kernel(torch, fn, steps: int, warmup: int, stream) -> float:
+    """Average ms per call of fn() over `steps` back-to-back calls, measured
+    with HIP events recorded on the stream the kernels are launched on."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    end.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(end) / steps
+
+
+def load_traffic(name: str):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, scaled by a copy kernel of known size); see profiles/."""
+    path = os.path.join(ROOT, "profiles", f"{name}_traffic.json")
+    try:
+        with open(path) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
+    """The reference's axpy on this host's cores over the same 2^24 vectors
+    (1024-element calls, as the reference kernel is written)."""
+    from oracle import ref
+
+    threads = min(os.cpu_count() or 1, 16)
+    if ref.available("ref_blas_scalar.so"):
+        isa = ref.best_isa()
+        ops = ref.BlasOperands(a_np, b_np, isa)
+        n = a_np.shape[0]
+        ref.blas("axpy", ops, q, mu, s, threads=threads)  # warm
+        t_single = ref.blas("axpy", ops, q, mu, s, threads=1)
+        passes, t0 = 0, time.perf_counter()
+        tot = 0.0
+        while passes < 2 or (time.perf_counter() - t0 < budget_s and passes < 50):
+            tot += ref.blas("axpy", ops, q, mu, s, threads=threads)
+            passes += 1
+        gbs = n * BYTES_PER_ELEM * passes / tot / 1e9
+        return {
+            "value": round(gbs, 3),
+            "unit": "GB/s",
+            "cores": threads,
+            "kind": "reference",
+            "sample": (f"reference blas_{isa}.c axpy over the 2^24-element vectors in 1024-element calls, "
+                       f"{passes} passes on {threads} threads ({ref.cpu_model()}); single-thread "
+                       f"{n * BYTES_PER_ELEM / t_single / 1e9:.3f} GB/s"),
+        }
+    import oracle
+
+    orc = oracle.load()
+    m = 1 << 20
+    t0 = time.perf_counter()
+    orc.axpy(a_np[:m], b_np[:m], s, q, mu)
+    dt = time.perf_counter() - t0
+    return {"value": round(m * BYTES_PER_ELEM / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["all", "axpy"], default="all",
+                    help="'axpy' times only the headline kernel (for profiling runs)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}, WORLD_SIZE {world}): "
+              "the transforms and vector ops are independent per element/transform, so a multi-GPU "
+              "run is N copies of this one; it is not launched here.", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import benchntt_amd as bn
+    from benchntt_amd import config as C
+    from benchntt_amd.residues import uniform
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    props = torch.cuda.get_device_properties(0)
+    log(f"device {props.name} ({getattr(props, 'gcnArchName', '?')}), {props.multi_processor_count} CUs")
+
+    n = 1 << 24
+    q, mu, s = C.Q_BLAS, C.MU_BLAS, C.AXPY_SCALAR
+    log("generating 3 x 2^24 seeded residues")
+    a_np, b_np = uniform(n, q, 1), uniform(n, q, 2)
+    da = torch.from_numpy(a_np.view(np.int64)).to(dev)
+    db = torch.from_numpy(b_np.view(np.int64)).to(dev)
+    dc = torch.empty_like(da)
+    torch.cuda.synchronize()
+
+    ms_axpy = time_kernel(torch, lambda: bn.axpy(dc, da, db, s, q, mu), args.steps, args.warmup, stream)
+    gbs = n * BYTES_PER_ELEM / (ms_axpy * 1e-3) / 1e9
+    log(f"axpy 2^24: {ms_axpy * 1e3:.1f} us/call, {gbs:.1f} GB/s")
+
+    extras = {}
+    if args.only == "all":
+        for name, fn in (("vvadd", lambda: bn.vector_add(dc, da, db, q)),
+                         ("vvsub", lambda: bn.vector_sub(dc, da, db, q)),
+                         ("vvmul", lambda: bn.vector_mul(dc, da, db, q, mu))):
+            ms = time_kernel(torch, fn, args.steps, args.warmup, stream)
+            extras[f"{name}_GBps"] = round(n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9, 1)
+            log(f"{name} 2^24: {ms * 1e3:.1f} us/call, {extras[f'{name}_GBps']} GB/s")
+
+        # NTT rows over q_N = 2^124 - 719 with a seeded random twiddle table
+        qn, mun = C.Q_NTT, C.MU_NTT
+        tw = torch.from_numpy(uniform(1024, qn, 4).view(np.int64)).to(dev)
+        x1 = torch.from_numpy(uniform(1024, qn, 3).view(np.int64)).to(dev)
+        y1 = torch.empty_like(x1)
+        ms1 = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), max(args.steps, 200), args.warmup, stream)
+        extras["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
+        extras["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
+        cnt = 8192
+        xb = torch.from_numpy(uniform(cnt * 1024, qn, 5).view(np.int64)).to(dev)
+        yb = torch.empty_like(xb)
+        msb = time_kernel(torch, lambda: bn.ntt(yb, xb, qn, mun, tw), max(args.steps // 4, 20), 5, stream)
+        extras["ntt_uniform_8192x1024_NTTs_per_s"] = round(cnt / (msb * 1e-3), 1)
+        extras["ntt_uniform_8192x1024_Gbutterflies_per_s"] = round(cnt * 5120 / (msb * 1e-3) / 1e9, 2)
+        extras["ntt_uniform_vs_plain_launch"] = round((cnt / (msb * 1e-3)) / (1.0 / (ms1 * 1e-3)), 1)
+        log(f"ntt plain 1024: {ms1 * 1e3:.2f} us; uniform 8192x1024: {msb:.3f} ms "
+            f"({extras['ntt_uniform_8192x1024_NTTs_per_s']:.0f} NTT/s)")
+
+    traffic = load_traffic("axpy_2_24")
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        cpu = cpu_baseline(a_np, b_np, q, mu, s)
+        log(f"cpu baseline: {cpu['value']} GB/s on {cpu['cores']} cores ({cpu['kind']})")
+
+    line = {
+        "metric": "NTTs/sec batched mixed-N vs per-NTT launch; axpy HBM GB/s; p99 latency",
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_axpy, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u128",
+        "data": "synthetic (seeded uniform residues in [0, q))",
+        "config": {"workload": "axpy c = a + s*b mod (2^124-59), 2^24 residues of 128 bits per vector "
+                               "(256 MiB each), inputs resident in HBM"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extras": extras,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,98 @@
+"""ctypes binding of libbntt.so (the C ABI of include/bntt.h).
+
+The library is built in-tree by ``python -m benchntt_amd.build``.  Loading fails
+loudly when it is missing: there is no CPU fallback for anything here.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbntt.so")
+
+BNTT_OK = 0
+
+
+class BnttError(RuntimeError):
+    """A bntt_* entry point returned an error code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bntt error {code}: {msg}")
+        self.code = code
+
+
+class U128(ctypes.Structure):
+    """bntt_u128: a 128-bit value in native unsigned __int128 layout."""
+
+    _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_uint64)]
+
+    @classmethod
+    def of(cls, v: int) -> "U128":
+        if v < 0 or v >= 1 << 128:
+            raise ValueError("value does not fit in 128 bits")
+        return cls(v & (2**64 - 1), v >> 64)
+
+
+_P = ctypes.c_void_p
+_PU = ctypes.POINTER(U128)
+_SZ = ctypes.c_size_t
+
+# name -> argtypes (all return int status, except where noted)
+_SIGS = {
+    "bntt_version": [],
+    "bntt_init": [],
+    "bntt_vvadd_dev": [_P, _P, _P, _SZ, _PU, _P],
+    "bntt_vvsub_dev": [_P, _P, _P, _SZ, _PU, _P],
+    "bntt_vvmul_dev": [_P, _P, _P, _SZ, _PU, _PU, _P],
+    "bntt_axpy_dev": [_P, _P, _P, _PU, _SZ, _PU, _PU, _P],
+    "bntt_ntt_fwd_dev": [_P, _P, _P, ctypes.c_int, _SZ, _PU, _PU, _P, _P],
+    "bntt_vvadd": [_P, _P, _P, _SZ, _PU],
+    "bntt_vvsub": [_P, _P, _P, _SZ, _PU],
+    "bntt_vvmul": [_P, _P, _P, _SZ, _PU, _PU],
+    "bntt_axpy": [_P, _P, _P, _PU, _SZ, _PU, _PU],
+    "bntt_ntt_fwd": [_P, _P, _P, ctypes.c_int, _PU, _PU],
+}
+# entry points that do not return a status
+_OTHER = {
+    "bntt_last_error": ([], ctypes.c_char_p),
+    "bntt_ntt_tmp_bytes": ([ctypes.c_int, _SZ], _SZ),
+}
+
+EXPORTS = tuple(_SIGS) + tuple(_OTHER)
+
+_lib: ctypes.CDLL | None = None
+
+
+def load(path: str | None = None) -> ctypes.CDLL:
+    """Load libbntt.so once and attach prototypes."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise ImportError(
+            f"{p} is missing: build the HIP library first (python -m benchntt_amd.build)"
+        )
+    lib = ctypes.CDLL(p)
+    for name, args in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    for name, (args, res) in _OTHER.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != BNTT_OK:
+        msg = load().bntt_last_error()
+        raise BnttError(rc, msg.decode() if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args))

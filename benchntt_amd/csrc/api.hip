@@ -1,0 +1,316 @@
+// api.hip -- the C ABI declared in include/bntt.h.
+//
+// Host-side responsibilities: argument checks, building the modulus
+// descriptor (including the check that selects the fast exact reduction),
+// error reporting, and the staging buffers of the host-pointer entry points.
+// All arithmetic happens in the kernels of blas.hip and ntt.hip; there is no
+// CPU path.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/bntt.h"
+#include "kernels.hpp"
+
+using bntt::Mod;
+using bntt::U128;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(BNTT_EHIP, "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+}
+
+typedef unsigned __int128 u128;
+
+u128 to_u128(const bntt_u128 &a) { return ((u128)a.hi << 64) | a.lo; }
+
+// floor(2^251 / m) for 2^123 < m < 2^124 by restoring division (the quotient
+// is below 2^128).
+u128 barrett_mu_251(u128 m)
+{
+    u128 r = 0, q = 0;
+    for (int i = 251; i >= 0; i--) {
+        // r < m < 2^124, so the shift cannot overflow
+        r = (r << 1) | (i == 251 ? 1 : 0);
+        q <<= 1;
+        if (r >= m) {
+            r -= m;
+            q |= 1;
+        }
+    }
+    return q;
+}
+
+Mod make_mod(const bntt_u128 *m, const bntt_u128 *mu)
+{
+    Mod md;
+    md.m = bntt::make_u128(m->lo, m->hi);
+    md.mu = bntt::make_u128(mu ? mu->lo : 0, mu ? mu->hi : 0);
+    md.notm = bntt::make_u128(~m->lo, ~m->hi);
+    md.c = 0;
+    md.pm = 0;
+    const u128 mm = to_u128(*m);
+    const u128 two124 = (u128)1 << 124;
+    if (mu && mm > ((u128)1 << 123) && mm < two124 && two124 - mm < 65536) {
+        if (barrett_mu_251(mm) == to_u128(*mu)) {
+            md.c = (uint32_t)(two124 - mm);
+            md.pm = 1;
+        }
+    }
+    return md;
+}
+
+std::mutex g_init_mu;
+int g_init_dev = -1;
+
+int ensure_init()
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    if (g_init_dev == dev) return BNTT_OK;
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(BNTT_ENODEV, "device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
+    g_init_dev = dev;
+    return BNTT_OK;
+}
+
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+int check_vec(const void *c, const void *a, const void *b, const bntt_u128 *m)
+{
+    if (!c || !a || !b || !m) return fail(BNTT_EINVAL, "null pointer argument");
+    if (!aligned16(c) || !aligned16(a) || !aligned16(b))
+        return fail(BNTT_EINVAL, "device arrays must be 16-byte aligned");
+    return BNTT_OK;
+}
+
+int blas_dev(int op, void *c, const void *a, const void *b, const bntt_u128 *x, size_t n, const bntt_u128 *m,
+             const bntt_u128 *mu, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n == 0) return BNTT_OK;
+    rc = check_vec(c, a, b, m);
+    if (rc) return rc;
+    if ((op == bntt::OP_VVMUL || op == bntt::OP_AXPY) && !mu) return fail(BNTT_EINVAL, "mu is required");
+    if (op == bntt::OP_AXPY && !x) return fail(BNTT_EINVAL, "axpy scalar is required");
+    const Mod md = make_mod(m, mu);
+    const U128 xs = x ? bntt::make_u128(x->lo, x->hi) : bntt::make_u128(0, 0);
+    hipError_t e = bntt::launch_blas(op, static_cast<U128 *>(c), static_cast<const U128 *>(a),
+                                     static_cast<const U128 *>(b), xs, n, md, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "blas kernel launch");
+    return BNTT_OK;
+}
+
+// Library-owned staging for the host-pointer entry points: one stream and one
+// grow-only device arena per process (current device at first use).
+struct Staging {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    void *buf = nullptr;
+    size_t cap = 0;
+    int dev = -1;
+};
+Staging g_stage;
+
+int stage_reserve(size_t bytes)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (g_stage.dev != dev) {
+        // different device than the arena's: start a new arena (the old one
+        // stays owned by its device until the process ends)
+        g_stage.stream = nullptr;
+        g_stage.buf = nullptr;
+        g_stage.cap = 0;
+        g_stage.dev = dev;
+    }
+    if (!g_stage.stream) {
+        e = hipStreamCreate(&g_stage.stream);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+    }
+    if (g_stage.cap < bytes) {
+        if (g_stage.buf) {
+            e = hipStreamSynchronize(g_stage.stream);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            (void)hipFree(g_stage.buf);
+            g_stage.buf = nullptr;
+            g_stage.cap = 0;
+        }
+        e = hipMalloc(&g_stage.buf, bytes);
+        if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+        g_stage.cap = bytes;
+    }
+    return BNTT_OK;
+}
+
+int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, const bntt_u128 *x, size_t n,
+              const bntt_u128 *m, const bntt_u128 *mu)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (n == 0) return BNTT_OK;
+    if (!c || !a || !b || !m) return fail(BNTT_EINVAL, "null pointer argument");
+    const size_t bytes = n * sizeof(bntt_u128);
+    std::lock_guard<std::mutex> lk(g_stage.mu);
+    rc = stage_reserve(3 * bytes);
+    if (rc) return rc;
+    char *base = static_cast<char *>(g_stage.buf);
+    void *da = base, *db = base + bytes, *dc = base + 2 * bytes;
+    hipStream_t s = g_stage.stream;
+    hipError_t e = hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+    rc = blas_dev(op, dc, da, db, x, n, m, mu, s);
+    if (rc) return rc;
+    e = hipMemcpyAsync(c, dc, bytes, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return BNTT_OK;
+}
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+int bntt_version(void) { return 100; }
+
+int bntt_init(void) { return ensure_init(); }
+
+const char *bntt_last_error(void) { return g_err.c_str(); }
+
+int bntt_vvadd_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, void *stream)
+{
+    return blas_dev(bntt::OP_VVADD, c, a, b, nullptr, n, m, nullptr, stream);
+}
+
+int bntt_vvsub_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, void *stream)
+{
+    return blas_dev(bntt::OP_VVSUB, c, a, b, nullptr, n, m, nullptr, stream);
+}
+
+int bntt_vvmul_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, const bntt_u128 *mu,
+                   void *stream)
+{
+    return blas_dev(bntt::OP_VVMUL, c, a, b, nullptr, n, m, mu, stream);
+}
+
+int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, size_t n, const bntt_u128 *m,
+                  const bntt_u128 *mu, void *stream)
+{
+    return blas_dev(bntt::OP_AXPY, c, a, b, x, n, m, mu, stream);
+}
+
+size_t bntt_ntt_tmp_bytes(int log2n, size_t count)
+{
+    if (log2n <= bntt::kLdsMaxLog2 || log2n > bntt::kMaxLog2) return 0;
+    return (count << log2n) * sizeof(bntt_u128);
+}
+
+int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t count, const bntt_u128 *m,
+                     const bntt_u128 *mu, void *tmp, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2)
+        return fail(BNTT_EINVAL, "log2n = %d outside [%d, %d]", log2n, bntt::kMinLog2, bntt::kMaxLog2);
+    if (count == 0) return BNTT_OK;
+    if (!y || !x || !tw || !m || !mu) return fail(BNTT_EINVAL, "null pointer argument");
+    if (!aligned16(y) || !aligned16(x) || !aligned16(tw) || (tmp && !aligned16(tmp)))
+        return fail(BNTT_EINVAL, "device arrays must be 16-byte aligned");
+    const size_t n = (size_t)1 << log2n;
+    if (log2n > bntt::kLdsMaxLog2) {
+        if (!tmp) return fail(BNTT_EINVAL, "log2n = %d needs a tmp buffer of bntt_ntt_tmp_bytes()", log2n);
+        const char *yb = static_cast<const char *>(y), *xb = static_cast<const char *>(x);
+        const size_t bytes = count * n * sizeof(bntt_u128);
+        if (yb < xb + bytes && xb < yb + bytes) return fail(BNTT_EINVAL, "y must not overlap x for log2n > 12");
+    }
+    const Mod md = make_mod(m, mu);
+    hipError_t e = bntt::launch_ntt_uniform(static_cast<U128 *>(y), static_cast<const U128 *>(x),
+                                            static_cast<const U128 *>(tw), log2n, count, md,
+                                            static_cast<U128 *>(tmp), static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
+    return BNTT_OK;
+}
+
+int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m)
+{
+    return blas_host(bntt::OP_VVADD, c, a, b, nullptr, n, m, nullptr);
+}
+
+int bntt_vvsub(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m)
+{
+    return blas_host(bntt::OP_VVSUB, c, a, b, nullptr, n, m, nullptr);
+}
+
+int bntt_vvmul(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m,
+               const bntt_u128 *mu)
+{
+    return blas_host(bntt::OP_VVMUL, c, a, b, nullptr, n, m, mu);
+}
+
+int bntt_axpy(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, const bntt_u128 *x, size_t n,
+              const bntt_u128 *m, const bntt_u128 *mu)
+{
+    return blas_host(bntt::OP_AXPY, c, a, b, x, n, m, mu);
+}
+
+int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2n, const bntt_u128 *m,
+                 const bntt_u128 *mu)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2)
+        return fail(BNTT_EINVAL, "log2n = %d outside [%d, %d]", log2n, bntt::kMinLog2, bntt::kMaxLog2);
+    if (!Y || !X || !tw || !m || !mu) return fail(BNTT_EINVAL, "null pointer argument");
+    const size_t n = (size_t)1 << log2n;
+    const size_t bytes = n * sizeof(bntt_u128);
+    const size_t tmpb = bntt_ntt_tmp_bytes(log2n, 1);
+    std::lock_guard<std::mutex> lk(g_stage.mu);
+    rc = stage_reserve(3 * bytes + tmpb);
+    if (rc) return rc;
+    char *base = static_cast<char *>(g_stage.buf);
+    void *dx = base, *dtw = base + bytes, *dy = base + 2 * bytes;
+    void *dtmp = tmpb ? base + 3 * bytes : nullptr;
+    hipStream_t s = g_stage.stream;
+    hipError_t e = hipMemcpyAsync(dx, X, bytes, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dtw, tw, bytes, hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+    rc = bntt_ntt_fwd_dev(dy, dx, dtw, log2n, 1, m, mu, dtmp, s);
+    if (rc) return rc;
+    e = hipMemcpyAsync(Y, dy, bytes, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync D2H");
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    return BNTT_OK;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

@@ -1,0 +1,310 @@
+// u128.hpp -- 128-bit modular arithmetic for gfx950 (also compiles for the host).
+//
+// A residue is four 32-bit limbs, least significant first: the native memory
+// layout of unsigned __int128, so one element is one 16-byte vector load
+// (global_load_dwordx4 / ds_read_b128).  Products are built from 32x32->64
+// multiply-adds, which the compiler maps to v_mad_u64_u32; carries use the
+// add/sub-with-carry builtins, which map to v_add_co/v_addc/v_sub_co/v_subb.
+//
+// Semantics are those of the reference's scalar primitives, bit for bit and
+// for every 128-bit input (citations are to the reference's src/):
+//   addmod_ref   ntt_scalar.c:49-79   literal, including its edge behaviour
+//                (when lo64(a+b) == lo64(m) and a+b >= m the high word borrows
+//                from !c5, so the result is a+b-m-2^64, or m when a+b == m)
+//   submod_ref   ntt_scalar.c:81-104  literal
+//   mulmod_ref   ntt_scalar.c:106-199 literal Barrett with the MBITS=124 shift
+//                schedule, including the 128-bit wraps of mul128/mulhi128
+//   mulmod_pm    exact a*b mod q for q = 2^124 - c, c < 2^16, a,b < 2^124.
+//                When mu = floor(2^251/q) and q is in (2^123, 2^124), the
+//                reference's Barrett quotient is short by at most one, so its
+//                single conditional subtract returns exactly a*b mod q for all
+//                a, b < 2^124; mulmod_pm returns the same value with about half
+//                the multiplies (two folds of 2^124 = c mod q).
+//   mulmod       mulmod_pm when the modulus qualifies and both operands are
+//                below 2^124, mulmod_ref otherwise: bit-exact in every case.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bntt {
+
+#define BNTT_HD __host__ __device__ __forceinline__
+
+struct alignas(16) U128 {
+    uint32_t v[4];
+};
+
+// Modulus descriptor passed to kernels by value.
+struct Mod {
+    U128 m;       // modulus
+    U128 mu;      // Barrett constant of the reference formula
+    U128 notm;    // ~m (used by the literal addmod)
+    uint32_t c;   // 2^124 - m when pm != 0
+    uint32_t pm;  // m = 2^124 - c, c < 2^16, mu = floor(2^251/m)
+};
+
+BNTT_HD U128 make_u128(uint64_t lo, uint64_t hi)
+{
+    U128 r;
+    r.v[0] = (uint32_t)lo;
+    r.v[1] = (uint32_t)(lo >> 32);
+    r.v[2] = (uint32_t)hi;
+    r.v[3] = (uint32_t)(hi >> 32);
+    return r;
+}
+
+BNTT_HD uint64_t lo64(const U128 &a) { return (uint64_t)a.v[0] | ((uint64_t)a.v[1] << 32); }
+BNTT_HD uint64_t hi64(const U128 &a) { return (uint64_t)a.v[2] | ((uint64_t)a.v[3] << 32); }
+
+// 32-bit add/sub with carry (v_add_co_u32 / v_addc_co_u32 / v_sub_co_u32 / v_subb_co_u32).
+BNTT_HD uint32_t addc(uint32_t a, uint32_t b, uint32_t cin, uint32_t &cout)
+{
+    unsigned int co;
+    uint32_t r = __builtin_addc(a, b, cin, &co);
+    cout = co;
+    return r;
+}
+BNTT_HD uint32_t subb(uint32_t a, uint32_t b, uint32_t bin, uint32_t &bout)
+{
+    unsigned int bo;
+    uint32_t r = __builtin_subc(a, b, bin, &bo);
+    bout = bo;
+    return r;
+}
+
+// (hi:lo) >> s for 0 < s < 32 (v_alignbit_b32).
+BNTT_HD uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s)
+{
+    return (uint32_t)((((uint64_t)hi << 32) | lo) >> s);
+}
+
+BNTT_HD uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
+
+// ---------------------------------------------------------------- literal add/sub
+
+// ntt_scalar.c:49-79, literal.  notm = ~m.
+BNTT_HD U128 addmod_ref(const U128 &a, const U128 &b, const U128 &m, const U128 &notm)
+{
+    uint32_t k, s0, s1, s2, s3;
+    s0 = addc(a.v[0], b.v[0], 0, k);
+    s1 = addc(a.v[1], b.v[1], k, k);
+    s2 = addc(a.v[2], b.v[2], k, k);
+    s3 = addc(a.v[3], b.v[3], k, k);
+    // c5 = borrow of (ml - t18), i.e. t18 > ml
+    uint32_t c5;
+    (void)subb(m.v[0], s0, 0, c5);
+    (void)subb(m.v[1], s1, c5, c5);
+    // d4 = t18 - ml (mod 2^64)
+    uint32_t bb, d0, d1, d2, d3;
+    d0 = subb(s0, m.v[0], 0, bb);
+    d1 = subb(s1, m.v[1], bb, bb);
+    // d6 = t19 - mh - !c5 = t19 + ~mh + c5 (mod 2^64)
+    d2 = addc(s2, notm.v[2], c5, bb);
+    d3 = addc(s3, notm.v[3], bb, bb);
+    // i14 = borrow of (mh - t19 - c5) over 128 bits, i.e. t19 + c5 > mh
+    uint32_t i14;
+    (void)subb(m.v[2], s2, c5, i14);
+    (void)subb(m.v[3], s3, i14, i14);
+    U128 r;
+    r.v[0] = i14 ? d0 : s0;
+    r.v[1] = i14 ? d1 : s1;
+    r.v[2] = i14 ? d2 : s2;
+    r.v[3] = i14 ? d3 : s3;
+    return r;
+}
+
+// ntt_scalar.c:81-104, literal: a - b, plus m on a borrow out of 128 bits.
+BNTT_HD U128 submod_ref(const U128 &a, const U128 &b, const U128 &m)
+{
+    uint32_t k, d0, d1, d2, d3, e0, e1, e2, e3, kk;
+    d0 = subb(a.v[0], b.v[0], 0, k);
+    d1 = subb(a.v[1], b.v[1], k, k);
+    d2 = subb(a.v[2], b.v[2], k, k);
+    d3 = subb(a.v[3], b.v[3], k, k);
+    e0 = addc(d0, m.v[0], 0, kk);
+    e1 = addc(d1, m.v[1], kk, kk);
+    e2 = addc(d2, m.v[2], kk, kk);
+    e3 = addc(d3, m.v[3], kk, kk);
+    U128 r;
+    r.v[0] = k ? e0 : d0;
+    r.v[1] = k ? e1 : d1;
+    r.v[2] = k ? e2 : d2;
+    r.v[3] = k ? e3 : d3;
+    return r;
+}
+
+// ------------------------------------------------------------------- products
+
+// Full 256-bit product of two 128-bit values (schoolbook, 16 multiply-adds).
+// Exact for all inputs.
+BNTT_HD void mul_4x4(const U128 &a, const U128 &b, uint32_t p[8])
+{
+    uint64_t t;
+    t = (uint64_t)a.v[0] * b.v[0];
+    p[0] = (uint32_t)t;
+    t = mad32(a.v[0], b.v[1], t >> 32);
+    p[1] = (uint32_t)t;
+    t = mad32(a.v[0], b.v[2], t >> 32);
+    p[2] = (uint32_t)t;
+    t = mad32(a.v[0], b.v[3], t >> 32);
+    p[3] = (uint32_t)t;
+    p[4] = (uint32_t)(t >> 32);
+#pragma unroll
+    for (int i = 1; i < 4; i++) {
+        t = mad32(a.v[i], b.v[0], p[i]);
+        p[i] = (uint32_t)t;
+        t = mad32(a.v[i], b.v[1], (uint64_t)p[i + 1] + (t >> 32));
+        p[i + 1] = (uint32_t)t;
+        t = mad32(a.v[i], b.v[2], (uint64_t)p[i + 2] + (t >> 32));
+        p[i + 2] = (uint32_t)t;
+        t = mad32(a.v[i], b.v[3], (uint64_t)p[i + 3] + (t >> 32));
+        p[i + 3] = (uint32_t)t;
+        p[i + 4] = (uint32_t)(t >> 32);
+    }
+}
+
+// ---------------------------------------------------------- fast exact mulmod
+
+// a*b mod q for q = 2^124 - c, c < 2^16, a, b < 2^124.  Canonical result.
+//   x = a*b < 2^248 = xh*2^124 + xl
+//   t = xl + c*xh            < 2^124 + 2^140, t >> 124 <= c
+//   u = (t mod 2^124) + c*(t >> 124)  < 2^124 + 2^32 < 2q
+//   r = u >= q ? u - q : u, with u >= q  <=>  u + c >= 2^124.
+BNTT_HD U128 mulmod_pm(const U128 &a, const U128 &b, uint32_t c)
+{
+    uint32_t p[8];
+    mul_4x4(a, b, p);
+    const uint32_t h0 = funnel(p[4], p[3], 28), h1 = funnel(p[5], p[4], 28);
+    const uint32_t h2 = funnel(p[6], p[5], 28), h3 = funnel(p[7], p[6], 28);
+    uint64_t t = mad32(c, h0, p[0]);
+    const uint32_t t0 = (uint32_t)t;
+    t = mad32(c, h1, (uint64_t)p[1] + (t >> 32));
+    const uint32_t t1 = (uint32_t)t;
+    t = mad32(c, h2, (uint64_t)p[2] + (t >> 32));
+    const uint32_t t2 = (uint32_t)t;
+    t = mad32(c, h3, (uint64_t)(p[3] & 0x0FFFFFFFu) + (t >> 32));
+    const uint32_t th = (uint32_t)(t >> 28);  // <= c < 2^16
+    const uint32_t t3 = (uint32_t)t & 0x0FFFFFFFu;
+    uint32_t k, u0, u1, u2, u3;
+    u0 = addc(t0, c * th, 0, k);  // c*th < 2^32
+    u1 = addc(t1, 0, k, k);
+    u2 = addc(t2, 0, k, k);
+    u3 = t3 + k;  // < 2^28 + 1
+    uint32_t d0, d1, d2, d3;
+    d0 = addc(u0, c, 0, k);
+    d1 = addc(u1, 0, k, k);
+    d2 = addc(u2, 0, k, k);
+    d3 = u3 + k;
+    const bool ge = (d3 >> 28) != 0;
+    U128 r;
+    r.v[0] = ge ? d0 : u0;
+    r.v[1] = ge ? d1 : u1;
+    r.v[2] = ge ? d2 : u2;
+    r.v[3] = ge ? (d3 & 0x0FFFFFFFu) : u3;
+    return r;
+}
+
+// ---------------------------------------------------- literal reference Barrett
+
+struct W128 {
+    uint64_t lo, hi;
+};
+
+BNTT_HD W128 mul64w(uint64_t a, uint64_t b)
+{
+    const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32), b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    const uint64_t p00 = (uint64_t)a0 * b0, p01 = (uint64_t)a0 * b1;
+    const uint64_t p10 = (uint64_t)a1 * b0, p11 = (uint64_t)a1 * b1;
+    const uint64_t mid = (p00 >> 32) + (uint32_t)p01 + (uint32_t)p10;
+    W128 r;
+    r.lo = (mid << 32) | (uint32_t)p00;
+    r.hi = p11 + (p01 >> 32) + (p10 >> 32) + (mid >> 32);
+    return r;
+}
+
+BNTT_HD W128 add128(W128 a, W128 b)
+{
+    W128 r;
+    r.lo = a.lo + b.lo;
+    r.hi = a.hi + b.hi + (r.lo < a.lo ? 1u : 0u);
+    return r;
+}
+
+BNTT_HD W128 sub128(W128 a, W128 b)
+{
+    W128 r;
+    r.lo = a.lo - b.lo;
+    r.hi = a.hi - b.hi - (a.lo < b.lo ? 1u : 0u);
+    return r;
+}
+
+BNTT_HD bool lt128(W128 a, W128 b) { return a.hi < b.hi || (a.hi == b.hi && a.lo < b.lo); }
+
+// ntt_scalar.c:106-128 (mul128): the cross sum wraps at 2^128.
+BNTT_HD void mul128_ref(W128 &ch, W128 &cl, W128 a, W128 b)
+{
+    const W128 albl = mul64w(a.lo, b.lo), albh = mul64w(a.lo, b.hi);
+    const W128 ahbl = mul64w(a.hi, b.lo), ahbh = mul64w(a.hi, b.hi);
+    const W128 cx = add128(albh, ahbl);
+    const W128 cx2 = add128(cx, W128{albl.hi, 0});
+    ch = add128(ahbh, W128{cx2.hi, 0});
+    cl = W128{albl.lo, cx2.lo};
+}
+
+// ntt_scalar.c:130-149 (mullo128).
+BNTT_HD W128 mullo128_ref(W128 a, W128 b)
+{
+    const W128 albl = mul64w(a.lo, b.lo);
+    const uint64_t cx = a.lo * b.hi + a.hi * b.lo;
+    return W128{albl.lo, cx + albl.hi};
+}
+
+// ntt_scalar.c:151-171 (mulhi128).
+BNTT_HD W128 mulhi128_ref(W128 a, W128 b)
+{
+    const uint64_t alblh = mul64w(a.lo, b.lo).hi;
+    const W128 albh = mul64w(a.lo, b.hi), ahbl = mul64w(a.hi, b.lo), ahbh = mul64w(a.hi, b.hi);
+    const W128 cx = add128(albh, ahbl);
+    const W128 cx2 = add128(cx, W128{alblh, 0});
+    return add128(ahbh, W128{cx2.hi, 0});
+}
+
+// ntt_scalar.c:173-199 (mulmod128), MBITS = 124: shifts 122 and 6.
+BNTT_HD U128 mulmod_ref(const U128 &a_, const U128 &b_, const U128 &m_, const U128 &mu_)
+{
+    const W128 a{lo64(a_), hi64(a_)}, b{lo64(b_), hi64(b_)};
+    const W128 m{lo64(m_), hi64(m_)}, mu{lo64(mu_), hi64(mu_)};
+    W128 qh, ql;
+    mul128_ref(qh, ql, a, b);
+    const W128 tmpl0 = ql;
+    // ql = (ql >> 122) | (qh << 6)
+    W128 q;
+    q.lo = (ql.hi >> 58) | (qh.lo << 6);
+    q.hi = (qh.lo >> 58) | (qh.hi << 6);
+    const uint64_t qhq = qh.hi >> 58;
+    const W128 qmh1 = mulhi128_ref(q, mu);
+    const uint64_t qmh2 = qhq * mu.lo;
+    const W128 qmh = add128(qmh1, W128{qmh2, 0});
+    const W128 qq{(qmh.lo >> 1) | (qmh.hi << 63), qmh.hi >> 1};
+    const W128 qqml = mullo128_ref(qq, m);
+    const W128 tmpl = sub128(tmpl0, qqml);
+    const W128 r = lt128(tmpl, m) ? tmpl : sub128(tmpl, m);
+    return make_u128(r.lo, r.hi);
+}
+
+// --------------------------------------------------------------------- dispatch
+
+BNTT_HD bool below_2_124(const U128 &a, const U128 &b) { return ((a.v[3] | b.v[3]) >> 28) == 0; }
+
+template <bool PM>
+BNTT_HD U128 mulmod(const U128 &a, const U128 &b, const Mod &md)
+{
+    if (PM && below_2_124(a, b)) return mulmod_pm(a, b, md.c);
+    return mulmod_ref(a, b, md.m, md.mu);
+}
+
+BNTT_HD U128 addmod(const U128 &a, const U128 &b, const Mod &md) { return addmod_ref(a, b, md.m, md.notm); }
+BNTT_HD U128 submod(const U128 &a, const U128 &b, const Mod &md) { return submod_ref(a, b, md.m); }
+
+}  // namespace bntt

@@ -1,0 +1,149 @@
+"""Reference-shaped host interface over the C ABI.
+
+The functions mirror the reference's kernels (files of the reference's src/):
+    ntt(Y, X, modulus, mu, twiddles, as0, as1)   ntt_scalar.c:582-627
+    vector_add(c, a, b, m)                       blas_scalar.c:578-584
+    vector_sub(c, a, b, m)                       blas_scalar.c:586-592
+    vector_mul(c, a, b, m, mu)                   blas_scalar.c:594-600
+    axpy(c, a, b, x, m, mu)                      blas_scalar.c:602-608
+with the same argument order and meaning: outputs first, written in place,
+modulus/mu/scalars as Python ints.  Sizes come from the arrays instead of
+compile-time macros.  Arrays are either torch tensors on the GPU (the device
+entry points; work is enqueued on torch's current stream) or numpy arrays in
+host memory (the host entry points; synchronous).  Either way each element
+is 16 bytes in unsigned __int128 layout: numpy uint64 (n, 2) or torch int64
+(n, 2).  Errors raise BnttError; nothing runs on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import U128
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _nelem(a) -> int:
+    if _is_torch(a):
+        nbytes = a.numel() * a.element_size()
+    else:
+        nbytes = a.nbytes
+    if nbytes % 16:
+        raise ValueError("residue arrays hold 16 bytes per element")
+    return nbytes // 16
+
+
+def _dev_ptr(a) -> int:
+    if not a.is_cuda:
+        raise ValueError("torch tensors must be on the GPU")
+    if not a.is_contiguous():
+        raise ValueError("tensors must be contiguous")
+    return a.data_ptr()
+
+
+def _host_ptr(a: np.ndarray) -> int:
+    if not isinstance(a, np.ndarray) or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("host arrays must be C-contiguous numpy arrays")
+    return a.ctypes.data
+
+
+def _stream(stream=None):
+    if stream is not None:
+        return ctypes.c_void_p(int(stream))
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _mode(*arrays) -> str:
+    kinds = {_is_torch(a) for a in arrays}
+    if len(kinds) != 1:
+        raise ValueError("mixing torch tensors and numpy arrays")
+    return "dev" if kinds.pop() else "host"
+
+
+def _same_len(*arrays) -> int:
+    ns = {_nelem(a) for a in arrays}
+    if len(ns) != 1:
+        raise ValueError("arrays differ in length")
+    return ns.pop()
+
+
+def _blas(name, c, a, b, extra_before_n, m, mu, stream):
+    n = _same_len(c, a, b)
+    mm = U128.of(m)
+    args_mu = [ctypes.byref(U128.of(mu))] if mu is not None else []
+    if _mode(c, a, b) == "dev":
+        ptrs = [_dev_ptr(c), _dev_ptr(a), _dev_ptr(b)]
+        _lib.call(f"bntt_{name}_dev", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu, _stream(stream))
+    else:
+        ptrs = [_host_ptr(c), _host_ptr(a), _host_ptr(b)]
+        _lib.call(f"bntt_{name}", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu)
+
+
+def vector_add(c, a, b, m: int, stream=None) -> None:
+    """c = a + b mod m (blas_scalar.c:578-584)."""
+    _blas("vvadd", c, a, b, [], m, None, stream)
+
+
+def vector_sub(c, a, b, m: int, stream=None) -> None:
+    """c = a - b mod m (blas_scalar.c:586-592)."""
+    _blas("vvsub", c, a, b, [], m, None, stream)
+
+
+def vector_mul(c, a, b, m: int, mu: int, stream=None) -> None:
+    """c = a * b mod m (blas_scalar.c:594-600)."""
+    _blas("vvmul", c, a, b, [], m, mu, stream)
+
+
+def axpy(c, a, b, x: int, m: int, mu: int, stream=None) -> None:
+    """c = a + x * b mod m (blas_scalar.c:602-608)."""
+    xs = U128.of(x)
+    _blas("axpy", c, a, b, [ctypes.byref(xs)], m, mu, stream)
+
+
+def tmp_elements(log2n: int, count: int = 1) -> int:
+    """Residues of scratch the device NTT needs (0 for log2n <= 12)."""
+    return _lib.load().bntt_ntt_tmp_bytes(log2n, count) // 16
+
+
+def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) -> None:
+    """Forward NTT (ntt_scalar.c:582-627): Y = NTT(X) with the twiddle table
+    `twiddles` (N entries).  N = len(twiddles); X and Y hold a whole number of
+    transforms of N points (one, as in the reference, or a uniform batch).
+    as0/as1 are the reference's scratch arguments: on the device path as0
+    serves as the multi-pass scratch when N > 4096 (len >= len(X)); as1 is
+    accepted and unused."""
+    del as1
+    n = _nelem(twiddles)
+    log2n = n.bit_length() - 1
+    if n != 1 << log2n:
+        raise ValueError("N must be a power of two")
+    total = _same_len(Y, X)
+    if total % n:
+        raise ValueError("X must hold a whole number of transforms")
+    count = total // n
+    mm, mmu = U128.of(modulus), U128.of(mu)
+    if _mode(Y, X, twiddles) == "dev":
+        need = tmp_elements(log2n, count)
+        tmp = 0
+        if need:
+            if as0 is None:
+                import torch
+
+                as0 = torch.empty((need, 2), dtype=torch.int64, device=X.device)
+            if _nelem(as0) < need:
+                raise ValueError(f"as0 must hold {need} residues for this N")
+            tmp = _dev_ptr(as0)
+        _lib.call("bntt_ntt_fwd_dev", _dev_ptr(Y), _dev_ptr(X), _dev_ptr(twiddles), log2n, count,
+                  ctypes.byref(mm), ctypes.byref(mmu), tmp, _stream(stream))
+    else:
+        for i in range(count):
+            off = i * n * 16
+            _lib.call("bntt_ntt_fwd", _host_ptr(Y) + off, _host_ptr(X) + off, _host_ptr(twiddles), log2n,
+                      ctypes.byref(mm), ctypes.byref(mmu))

@@ -1,0 +1,108 @@
+/*
+ * bntt.h -- C ABI of the MI355X (gfx950) benchNTT hot path: 128-bit modular
+ * forward NTT and vvadd / vvsub / vvmul / axpy mod q.
+ *
+ * The reference has no library: its kernels are plain C functions inside six
+ * benchmark executables, with N a compile-time macro and the BLAS length fixed
+ * at 1024.  This header exports the same operations with the sizes as
+ * arguments; benchntt_compat.h maps the reference's own call shapes onto them.
+ * Entry point -> reference interface it replaces (files of the reference's src/):
+ *   bntt_ntt_fwd, bntt_ntt_fwd_dev   ntt()        ntt_scalar.c:582-627
+ *                                    (ntt_pre/ntt/ntt_post of ntt_avx2.c:507-655,
+ *                                     ntt_avx512.c:386-528: same network)
+ *   bntt_vvadd[_dev]                 vector_add() blas_scalar.c:578-584
+ *   bntt_vvsub[_dev]                 vector_sub() blas_scalar.c:586-592
+ *   bntt_vvmul[_dev]                 vector_mul() blas_scalar.c:594-600
+ *   bntt_axpy[_dev]                  axpy()       blas_scalar.c:602-608
+ *
+ * Data: a residue is 16 bytes in the native layout of unsigned __int128 (low
+ * 64-bit word first), exactly the reference's uint128_t arrays.  Device arrays
+ * must be 16-byte aligned.  Results are bit-identical to the reference's
+ * scalar functions (addmod128/submod128/mulmod128, ntt_scalar.c:49-199, with
+ * MBITS = 124) for every input, including their edge behaviour.
+ *
+ * Errors: every entry point returns BNTT_OK or a nonzero code; the message of
+ * the calling thread's last failure is in bntt_last_error().  Nothing falls
+ * back to the CPU: with no usable gfx950 device every call fails.
+ *
+ * _dev entry points take device pointers and enqueue on `stream` (a
+ * hipStream_t; NULL = the null stream) without synchronizing.  The others take
+ * host pointers, stage through library-owned device buffers and return when
+ * the results are in host memory.  Calls are thread-safe.
+ */
+#ifndef BNTT_H
+#define BNTT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* A 128-bit value in native unsigned __int128 layout. */
+typedef struct bntt_u128 {
+    uint64_t lo;
+    uint64_t hi;
+} bntt_u128;
+
+enum {
+    BNTT_OK = 0,
+    BNTT_EINVAL = 1, /* bad size, pointer or alignment */
+    BNTT_EHIP = 2,   /* HIP runtime error */
+    BNTT_ENODEV = 3, /* no gfx950 device */
+    BNTT_ENOMEM = 4  /* device allocation failed */
+};
+
+/* Version of this ABI (major*10000 + minor*100 + patch). */
+int bntt_version(void);
+
+/* Select the current HIP device and check it is gfx950.  Called implicitly by
+ * every entry point; calling it first moves start-up cost out of timed code. */
+int bntt_init(void);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *bntt_last_error(void);
+
+/* ------------------------------------------------------------ device pointers */
+
+/* c[i] = a[i] + b[i] mod m, i < n (ntt_scalar.c:49-79 semantics). */
+int bntt_vvadd_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, void *stream);
+/* c[i] = a[i] - b[i] mod m. */
+int bntt_vvsub_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, void *stream);
+/* c[i] = a[i] * b[i] mod m (Barrett with mu = floor(2^251 / m) in the reference). */
+int bntt_vvmul_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, const bntt_u128 *mu,
+                   void *stream);
+/* c[i] = a[i] + x * b[i] mod m (blas_scalar.c:602-608: the scalar multiplies b). */
+int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, size_t n, const bntt_u128 *m,
+                  const bntt_u128 *mu, void *stream);
+
+/* Forward NTT of `count` transforms of N = 2^log2n points, stored one after
+ * another in x (count*N residues) into y, all with the twiddle table tw
+ * (N residues; entry 0 unused, stage s uses entries [2^s, 2^(s+1))).
+ * 1 <= log2n <= 24.  For log2n <= 12 every transform is done in one launch and
+ * y may equal x.  For log2n > 12 the transform takes several passes through
+ * device memory: tmp must point to bntt_ntt_tmp_bytes(log2n, count) bytes of
+ * device memory and y must not overlap x.  count = 1 is the plain
+ * one-transform path. */
+int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t count, const bntt_u128 *m,
+                     const bntt_u128 *mu, void *tmp, void *stream);
+size_t bntt_ntt_tmp_bytes(int log2n, size_t count);
+
+/* -------------------------------------------------------------- host pointers */
+
+int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m);
+int bntt_vvsub(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m);
+int bntt_vvmul(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m,
+               const bntt_u128 *mu);
+int bntt_axpy(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, const bntt_u128 *x, size_t n,
+              const bntt_u128 *m, const bntt_u128 *mu);
+/* One forward NTT of 2^log2n points; Y and X are host arrays, X is not modified. */
+int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2n, const bntt_u128 *m,
+                 const bntt_u128 *mu);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BNTT_H */

@@ -1,0 +1,33 @@
+// Host build of the device arithmetic (benchntt_amd/csrc/u128.hpp) so that the
+// CPU test suite can compare it with the oracle before any GPU run.  Every
+// function of u128.hpp is __host__ __device__ and uses no device builtins.
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../benchntt_amd/csrc/u128.hpp"
+
+using bntt::U128;
+
+extern "C" {
+
+// op: 0 addmod_ref, 1 submod_ref, 2 mulmod_ref, 3 mulmod_pm, 4 mulmod<true>, 5 mulmod<false>
+void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U128 *m, const U128 *mu, uint32_t c)
+{
+    bntt::Mod md;
+    md.m = *m;
+    md.mu = *mu;
+    for (int i = 0; i < 4; i++) md.notm.v[i] = ~m->v[i];
+    md.c = c;
+    md.pm = c != 0;
+    for (size_t i = 0; i < n; i++) {
+        switch (op) {
+        case 0: out[i] = bntt::addmod(a[i], b[i], md); break;
+        case 1: out[i] = bntt::submod(a[i], b[i], md); break;
+        case 2: out[i] = bntt::mulmod_ref(a[i], b[i], md.m, md.mu); break;
+        case 3: out[i] = bntt::mulmod_pm(a[i], b[i], c); break;
+        case 4: out[i] = bntt::mulmod<true>(a[i], b[i], md); break;
+        default: out[i] = bntt::mulmod<false>(a[i], b[i], md); break;
+        }
+    }
+}
+}

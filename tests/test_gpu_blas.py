@@ -1,0 +1,116 @@
+"""GPU parity of vvadd / vvsub / vvmul / axpy against the oracle and the
+reference's golden vectors (bit-exact)."""
+import numpy as np
+import pytest
+
+from benchntt_amd import config as C
+from benchntt_amd.residues import uniform
+from conftest import from_dev, to_dev
+from test_u128_host import operands
+
+pytestmark = pytest.mark.gpu
+
+QB, MUB, S = C.Q_BLAS, C.MU_BLAS, C.AXPY_SCALAR
+
+
+def run_dev(torch, op, a, b, x=S, q=QB, mu=MUB):
+    import benchntt_amd as bn
+
+    da, db = to_dev(torch, a), to_dev(torch, b)
+    dc = torch.empty_like(da)
+    if op == "vvadd":
+        bn.vector_add(dc, da, db, q)
+    elif op == "vvsub":
+        bn.vector_sub(dc, da, db, q)
+    elif op == "vvmul":
+        bn.vector_mul(dc, da, db, q, mu)
+    else:
+        bn.axpy(dc, da, db, x, q, mu)
+    torch.cuda.synchronize()
+    return from_dev(dc)
+
+
+def want(orc, op, a, b, x=S, q=QB, mu=MUB):
+    if op == "vvadd":
+        return orc.vvadd(a, b, q)
+    if op == "vvsub":
+        return orc.vvsub(a, b, q)
+    if op == "vvmul":
+        return orc.vvmul(a, b, q, mu)
+    return orc.axpy(a, b, x, q, mu)
+
+
+OPS = ["vvadd", "vvsub", "vvmul", "axpy"]
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_golden_device(gpu, fixtures, op):
+    a, b = fixtures["twd_1024"], fixtures["x_1024"]
+    assert np.array_equal(run_dev(gpu, op, a, b), fixtures[f"{op}_1024"])
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_golden_host_entry(gpu, fixtures, op):
+    import benchntt_amd as bn
+
+    a, b = fixtures["twd_1024"].copy(), fixtures["x_1024"].copy()
+    c = np.zeros_like(a)
+    fn = {"vvadd": lambda: bn.vector_add(c, a, b, QB), "vvsub": lambda: bn.vector_sub(c, a, b, QB),
+          "vvmul": lambda: bn.vector_mul(c, a, b, QB, MUB), "axpy": lambda: bn.axpy(c, a, b, S, QB, MUB)}[op]
+    fn()
+    assert np.array_equal(c, fixtures[f"{op}_1024"])
+
+
+@pytest.mark.parametrize("n", [1, 3, 255, 256, 511, 513, 1000, 65537])
+@pytest.mark.parametrize("op", OPS)
+def test_ragged_sizes(gpu, orc, op, n):
+    a, b = uniform(n, QB, n), uniform(n, QB, n + 7)
+    assert np.array_equal(run_dev(gpu, op, a, b), want(orc, op, a, b))
+
+
+@pytest.mark.parametrize("op", OPS)
+def test_edge_operands(gpu, orc, op):
+    """Operands at and above q and 2^124, and the reference addmod's edge."""
+    a, b = operands(QB, 23)
+    assert np.array_equal(run_dev(gpu, op, a, b), want(orc, op, a, b))
+
+
+@pytest.mark.parametrize("op", ["vvadd", "vvsub", "vvmul", "axpy"])
+def test_other_modulus_literal_path(gpu, orc, op):
+    """A modulus outside the fast form (q_116 with the reference's unused
+    MBITS=116 constant) takes the literal reference formulas."""
+    q = C.Q_116
+    mu = (1 << 235) // q
+    a, b = uniform(4099, q, 1), uniform(4099, q, 2)
+    x = 12345678901234567890123
+    assert np.array_equal(run_dev(gpu, op, a, b, x, q, mu), want(orc, op, a, b, x, q, mu))
+
+
+def test_full_size_2_24(gpu, orc):
+    """The benchmark size: 2^24 residues per vector, every op bit-exact."""
+    n = 1 << 24
+    a, b = uniform(n, QB, 2024), uniform(n, QB, 2025)
+    for op in OPS:
+        assert np.array_equal(run_dev(gpu, op, a, b), want(orc, op, a, b)), op
+
+
+def test_zero_length(gpu):
+    import benchntt_amd as bn
+
+    e = gpu.empty((0, 2), dtype=gpu.int64, device="cuda")
+    bn.vector_add(e, e, e, QB)
+    bn.axpy(e, e, e, S, QB, MUB)
+    gpu.cuda.synchronize()
+
+
+def test_errors(gpu):
+    import benchntt_amd as bn
+
+    a = gpu.zeros((8, 2), dtype=gpu.int64, device="cuda")
+    b = gpu.zeros((9, 2), dtype=gpu.int64, device="cuda")
+    with pytest.raises(ValueError):
+        bn.vector_add(a, a, b, QB)
+    raw = gpu.zeros(8 * 16 + 8, dtype=gpu.uint8, device="cuda")
+    mis = raw[8:].view(gpu.int64)  # 8-byte aligned, not 16
+    with pytest.raises(bn.BnttError):
+        bn.vector_add(mis, mis, mis, QB)

@@ -1,0 +1,119 @@
+"""GPU parity of the forward NTT against the oracle and the reference's golden
+vectors (bit-exact), over the plain one-transform path and uniform batches."""
+import numpy as np
+import pytest
+
+from benchntt_amd import config as C
+from benchntt_amd.residues import uniform
+from conftest import from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+QN, MUN = C.Q_NTT, C.MU_NTT
+
+
+def ntt_dev(torch, x, tw, q=QN, mu=MUN, inplace=False):
+    import benchntt_amd as bn
+
+    dx, dtw = to_dev(torch, x), to_dev(torch, tw)
+    dy = dx if inplace else torch.empty_like(dx)
+    bn.ntt(dy, dx, q, mu, dtw)
+    torch.cuda.synchronize()
+    return from_dev(dy)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512, 1024])
+def test_golden_device(gpu, fixtures, n):
+    y = ntt_dev(gpu, fixtures[f"x_{n}"], fixtures[f"twd_{n}"])
+    assert np.array_equal(y, fixtures[f"ver_{n}"])
+
+
+@pytest.mark.parametrize("n", [16, 1024])
+def test_golden_host_entry(gpu, fixtures, n):
+    import benchntt_amd as bn
+
+    x, tw = fixtures[f"x_{n}"].copy(), fixtures[f"twd_{n}"].copy()
+    y = np.zeros_like(x)
+    bn.ntt(y, x, QN, MUN, tw)
+    assert np.array_equal(y, fixtures[f"ver_{n}"])
+
+
+@pytest.mark.parametrize("L", list(range(1, 17)))
+def test_random_batches(gpu, orc, L):
+    """Every size 2^1..2^16, a few transforms per launch (one workgroup may
+    hold several transforms, and the tail workgroup is partial)."""
+    n = 1 << L
+    count = 3 if L <= 12 else 2
+    x, tw = uniform(count * n, QN, 10 + L), uniform(n, QN, 40 + L)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
+@pytest.mark.parametrize("L", [17, 18, 20])
+def test_large_plain(gpu, orc, L):
+    n = 1 << L
+    x, tw = uniform(n, QN, L), uniform(n, QN, 2 * L)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
+def test_uniform_batch_8192x1024(gpu, orc):
+    """The uniform-batch workload: 8192 transforms of 1024 points."""
+    x, tw = uniform(8192 * 1024, QN, 3), uniform(1024, QN, 4)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
+@pytest.mark.parametrize("L", [4, 10, 12])
+def test_in_place(gpu, orc, L):
+    n = 1 << L
+    x, tw = uniform(5 * n, QN, 1), uniform(n, QN, 2)
+    assert np.array_equal(ntt_dev(gpu, x, tw, inplace=True), orc.ntt(x, tw, QN, MUN))
+
+
+def test_edge_inputs(gpu, orc):
+    """Inputs and twiddles at q, above q and above 2^124 follow the reference
+    formulas exactly (the fast reduction hands such operands to the literal
+    Barrett path)."""
+    n = 256
+    rng = np.random.default_rng(0)
+    x, tw = uniform(n, QN, 5), uniform(n, QN, 6)
+    x[::7, 1] = rng.integers(0, 2**64, size=x[::7].shape[0], dtype=np.uint64)
+    tw[::5, 1] |= np.uint64(1 << 60)
+    x[3] = [QN & (2**64 - 1), QN >> 64]
+    x[4] = [2**64 - 1, 2**64 - 1]
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
+def test_other_modulus_literal_path(gpu, orc):
+    q = C.Q_116
+    mu = (1 << 235) // q
+    for L in (6, 13):
+        x, tw = uniform(2 << L, q, L), uniform(1 << L, q, L + 1)
+        assert np.array_equal(ntt_dev(gpu, x, tw, q, mu), orc.ntt(x, tw, q, mu))
+
+
+def test_linearity_2_20(gpu):
+    """Size-independent property at 2^20 points: NTT(a + b) = NTT(a) + NTT(b)."""
+    import benchntt_amd as bn
+
+    n = 1 << 20
+    a, b, tw = uniform(n, QN, 1), uniform(n, QN, 2), uniform(n, QN, 3)
+    da, db, dtw = to_dev(gpu, a), to_dev(gpu, b), to_dev(gpu, tw)
+    s, fa, fb, fs, t = (gpu.empty_like(da) for _ in range(5))
+    bn.vector_add(s, da, db, QN)
+    bn.ntt(fa, da, QN, MUN, dtw)
+    bn.ntt(fb, db, QN, MUN, dtw)
+    bn.ntt(fs, s, QN, MUN, dtw)
+    bn.vector_add(t, fa, fb, QN)
+    gpu.cuda.synchronize()
+    assert gpu.equal(t, fs)
+
+
+def test_bad_sizes(gpu):
+    import benchntt_amd as bn
+
+    x = gpu.zeros((24, 2), dtype=gpu.int64, device="cuda")
+    tw = gpu.zeros((16, 2), dtype=gpu.int64, device="cuda")
+    with pytest.raises(ValueError):
+        bn.ntt(x, x, QN, MUN, tw)  # 24 is not a whole number of 16-point transforms
+    tw3 = gpu.zeros((3, 2), dtype=gpu.int64, device="cuda")
+    with pytest.raises(ValueError):
+        bn.ntt(x[:3], x[:3], QN, MUN, tw3)

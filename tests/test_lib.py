@@ -1,0 +1,80 @@
+"""CPU checks of the C ABI library: it loads, exports exactly what
+include/bntt.h declares, and fails loudly (no CPU fallback) without a GPU."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    syms = set()
+    for name in os.listdir(os.path.join(ROOT, "include")):
+        if not name.endswith(".h") or name == "benchntt_compat.h":
+            continue
+        text = open(os.path.join(ROOT, "include", name)).read()
+        syms |= set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(bntt_[a-z0-9_]+)\s*\(", text, re.M))
+    return syms
+
+
+def test_exports_match_header():
+    from benchntt_amd import _lib
+
+    lib_path = _lib.LIB_PATH
+    assert os.path.exists(lib_path)
+    out = subprocess.run(["nm", "-D", "--defined-only", lib_path], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    decl = declared_symbols()
+    assert decl, "no declarations found"
+    assert decl <= exported, decl - exported
+    assert {s for s in exported if s.startswith("bntt_")} == decl
+    assert set(_lib.EXPORTS) == decl
+
+
+def test_loads_and_versions():
+    from benchntt_amd import _lib
+
+    lib = _lib.load()
+    assert lib.bntt_version() == 100
+    assert lib.bntt_ntt_tmp_bytes(12, 5) == 0
+    assert lib.bntt_ntt_tmp_bytes(16, 3) == 3 * (1 << 16) * 16
+
+
+def test_no_cpu_fallback_without_gpu():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    import benchntt_amd as bn
+
+    a = np.zeros((16, 2), np.uint64)
+    with pytest.raises(bn.BnttError):
+        bn.vector_mul(a, a, a, bn.config.Q_BLAS, bn.config.MU_BLAS)
+    with pytest.raises(bn.BnttError):
+        bn.ntt(a, a, bn.config.Q_NTT, bn.config.MU_NTT, a)
+
+
+def test_compat_header_compiles(tmp_path):
+    """include/benchntt_compat.h gives C callers the reference's own call
+    shapes (ntt, vector_add, ...); a translation unit written like the
+    reference's drivers compiles and links against libbntt.so."""
+    src = tmp_path / "caller.c"
+    src.write_text(
+        "#define stages 10\n"
+        "#define N (1 << stages)\n"
+        '#include "benchntt_compat.h"\n'
+        "int main(void) {\n"
+        "  static uint128_t x[N], y[N], tw[N], as0[N], as1[N], c[1024];\n"
+        "  uint128_t m = MODULUS, mu = MU;\n"
+        "  ntt(y, x, m, mu, tw, as0, as1);\n"
+        "  vector_add(c, x, tw, m); vector_sub(c, x, tw, m);\n"
+        "  vector_mul(c, x, tw, m, mu); axpy(c, x, tw, m, m, mu);\n"
+        "  return 0;\n}\n"
+    )
+    lib_dir = os.path.join(ROOT, "benchntt_amd")
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", lib_dir, "-lbntt", "-Wl,-rpath," + lib_dir, "-o", str(tmp_path / "caller")],
+                   check=True)

@@ -1,0 +1,81 @@
+"""The device arithmetic of benchntt_amd/csrc/u128.hpp, compiled for the host,
+against the oracle on random and edge operands (CPU only; catches arithmetic
+mistakes before any GPU run)."""
+import ctypes
+import os
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+from benchntt_amd import config as C
+from benchntt_amd.residues import from_ints, uniform
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def chk(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("u128") / "u128_host_check.so")
+    src = os.path.join(HERE, "csrc", "u128_host_check.hip")
+    r = subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
+                        src, "-o", out], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    lib = ctypes.CDLL(out)
+    lib.chk_op.argtypes = [ctypes.c_int] + [ctypes.c_void_p] * 3 + [ctypes.c_size_t] + [ctypes.c_void_p] * 2 + [
+        ctypes.c_uint32]
+    return lib
+
+
+def run(lib, op, a, b, m, mu, c=0):
+    out = np.zeros_like(a)
+    mm, mmu = from_ints([m]), from_ints([mu])
+    lib.chk_op(op, out.ctypes.data, a.ctypes.data, b.ctypes.data, a.shape[0], mm.ctypes.data, mmu.ctypes.data, c)
+    return out
+
+
+def operands(q, seed):
+    rng = random.Random(seed)
+    ml = q & (2**64 - 1)
+    edges = [0, 1, q - 1, q - 2, q, q + 1, (1 << 124) - 1, 1 << 123, (1 << 64) - 1, 1 << 64, q // 2]
+    big = [rng.randrange(1 << 128) for _ in range(20)]
+    av, bv = [], []
+    for x in edges + big:
+        for y in edges + big:
+            av.append(x)
+            bv.append(y)
+    for _ in range(200):  # the reference addmod's edge: lo64(a+b) == lo64(q), a+b >= q
+        hi = rng.randrange(q >> 64, 2 * (q >> 64))
+        s = (hi << 64) | ml
+        x = rng.randrange(max(0, s - q + 1), min(q, s + 1))
+        av.append(x)
+        bv.append(s - x)
+    a = np.concatenate([from_ints(av), uniform(20000, q, seed)])
+    b = np.concatenate([from_ints(bv), uniform(20000, q, seed + 1)])
+    return a, b
+
+
+@pytest.mark.parametrize("q,mu", [(C.Q_NTT, C.MU_NTT), (C.Q_BLAS, C.MU_BLAS)])
+def test_device_arith_matches_oracle(chk, orc, q, mu):
+    a, b = operands(q, 17)
+    c = (1 << 124) - q
+    assert np.array_equal(run(chk, 0, a, b, q, mu), orc.vvadd(a, b, q))
+    assert np.array_equal(run(chk, 1, a, b, q, mu), orc.vvsub(a, b, q))
+    want = orc.vvmul(a, b, q, mu)
+    assert np.array_equal(run(chk, 2, a, b, q, mu), want)
+    assert np.array_equal(run(chk, 4, a, b, q, mu, c), want)  # fast path + fallback dispatch
+    assert np.array_equal(run(chk, 5, a, b, q, mu), want)
+    small = (a[:, 1] >> np.uint64(60) == 0) & (b[:, 1] >> np.uint64(60) == 0)
+    assert np.array_equal(run(chk, 3, a[small], b[small], q, mu, c), want[small])
+
+
+def test_device_arith_other_modulus(chk, orc):
+    """A modulus that is not of the form 2^124 - c (and a mu that is not
+    floor(2^251/m)) still follows the reference formulas literally."""
+    q = C.Q_116
+    mu = (1 << 235) // q
+    a, b = operands(q, 5)
+    assert np.array_equal(run(chk, 5, a, b, q, mu), orc.vvmul(a, b, q, mu))
+    assert np.array_equal(run(chk, 0, a, b, q, mu), orc.vvadd(a, b, q))
+    assert np.array_equal(run(chk, 1, a, b, q, mu), orc.vvsub(a, b, q))
