@@ -74,6 +74,15 @@ def load(path: str | None = None) -> ctypes.CDLL:
         raise ImportError(
             f"{p} is missing: build the HIP library first (python -m benchntt_amd.build)"
         )
+    # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.  Importing
+    # it first makes libbntt.so bind to those same copies (same SONAMEs), so the
+    # process has one HIP runtime and torch's device pointers and streams are
+    # valid in libbntt.  Loaded the other way round, the process would hold two
+    # HSA runtimes and the second one sees no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(p)
     for name, args in _SIGS.items():
         fn = getattr(lib, name)
