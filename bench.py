@@ -99,6 +99,56 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
             "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
 
 
+def uneven_batch(torch, np, bn, qn, mun, stream) -> dict:
+    """20,000 forward NTTs, N log-uniform on 2^4..2^16 in pairs (seeded), one
+    twiddle table per size, each pair's pointwise product after its two
+    transforms: one persistent work-queue kernel per level, replayed from a
+    hipGraph, against one launch sequence per transform from native code."""
+    from benchntt_amd.batch import BatchPlan, TaskList, launch_each, load_tune_table
+    from benchntt_amd.workloads import uneven_pairs
+
+    wl = uneven_pairs(20000, qn, seed=2024)
+    dev = torch.device("cuda:0")
+    dx = torch.from_numpy(wl.x.view(np.int64)).to(dev)
+    dy = torch.empty_like(dx)
+    dc = torch.empty((wl.pair_total, 2), dtype=torch.int64, device=dev)
+    tabs = {l: torch.from_numpy(t.view(np.int64)).to(dev) for l, t in wl.tables.items()}
+    tasks, muls = [], []
+    for l, off in zip(wl.log2n, wl.offsets):
+        n = 1 << int(l)
+        tasks.append((dx[off:off + n], dy[off:off + n], tabs[int(l)]))
+    for j, poff in enumerate(wl.pair_offsets):
+        n = 1 << int(wl.log2n[2 * j])
+        a0, b0 = wl.offsets[2 * j], wl.offsets[2 * j + 1]
+        muls.append((dy[a0:a0 + n], dy[b0:b0 + n], dc[poff:poff + n]))
+    tune = load_tune_table()
+    butterflies = float(sum((1 << int(l)) * int(l) // 2 for l in wl.log2n))
+    s2 = torch.cuda.Stream()
+    out = {"uneven_total_coefficients": wl.total, "uneven_tuned": bool(tune)}
+    with torch.cuda.stream(s2):
+        full = BatchPlan(tasks, muls, qn, mun, tune=tune, graph=True)
+        only = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True)
+        ms_full = time_kernel(torch, full.execute, 5, 2, s2)
+        ms_only = time_kernel(torch, only.execute, 5, 2, s2)
+        tmp = torch.empty((1 << 16, 2), dtype=torch.int64, device=dev)
+        tl = TaskList(tasks)
+        ms_each = time_kernel(torch, lambda: launch_each(tl, qn, mun, tmp), 1, 1, s2)
+    info = full.info()
+    full.close()
+    only.close()
+    out["uneven_batched_NTTs_per_s"] = round(20000 / (ms_only * 1e-3), 1)
+    out["uneven_batched_with_products_ms"] = round(ms_full, 3)
+    out["uneven_batched_with_products_NTTs_per_s"] = round(20000 / (ms_full * 1e-3), 1)
+    out["uneven_per_launch_NTTs_per_s"] = round(20000 / (ms_each * 1e-3), 1)
+    out["uneven_batched_vs_per_launch"] = round(ms_each / ms_only, 2)
+    out["uneven_Gbutterflies_per_s"] = round(butterflies / (ms_only * 1e-3) / 1e9, 2)
+    out["uneven_plan_levels"] = info["levels"]
+    log(f"uneven 20000: batched {ms_only:.3f} ms ({out['uneven_batched_NTTs_per_s']:.0f} NTT/s), "
+        f"with products {ms_full:.3f} ms, per-launch {ms_each:.1f} ms -> "
+        f"{out['uneven_batched_vs_per_launch']}x")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +217,8 @@ def main() -> int:
         extras["ntt_uniform_vs_plain_launch"] = round((cnt / (msb * 1e-3)) / (1.0 / (ms1 * 1e-3)), 1)
         log(f"ntt plain 1024: {ms1 * 1e3:.2f} us; uniform 8192x1024: {msb:.3f} ms "
             f"({extras['ntt_uniform_8192x1024_NTTs_per_s']:.0f} NTT/s)")
+        del xb, yb
+        extras.update(uneven_batch(torch, np, bn, qn, mun, stream))
 
     traffic = load_traffic("axpy_2_24")
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

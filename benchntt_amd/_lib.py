@@ -34,6 +34,36 @@ class U128(ctypes.Structure):
         return cls(v & (2**64 - 1), v >> 64)
 
 
+class NttTask(ctypes.Structure):
+    """bntt_ntt_task."""
+
+    _fields_ = [("x", ctypes.c_void_p), ("y", ctypes.c_void_p), ("tw", ctypes.c_void_p),
+                ("log2n", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class MulTask(ctypes.Structure):
+    """bntt_mul_task."""
+
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p), ("n", ctypes.c_uint64)]
+
+
+class TuneEntry(ctypes.Structure):
+    """bntt_tune_entry."""
+
+    _fields_ = [("log2n", ctypes.c_int32), ("elog", ctypes.c_int32), ("passes", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class BatchInfo(ctypes.Structure):
+    """bntt_batch_info."""
+
+    _fields_ = [("levels", ctypes.c_int32), ("reserved", ctypes.c_int32), ("items", ctypes.c_uint64),
+                ("tmp_bytes", ctypes.c_uint64), ("level_items", ctypes.c_uint64 * 8),
+                ("level_grid", ctypes.c_int32 * 8), ("level_lds_bytes", ctypes.c_uint32 * 8)]
+
+
+BNTT_PLAN_GRAPH = 1
+
 _P = ctypes.c_void_p
 _PU = ctypes.POINTER(U128)
 _SZ = ctypes.c_size_t
@@ -52,6 +82,13 @@ _SIGS = {
     "bntt_vvmul": [_P, _P, _P, _SZ, _PU, _PU],
     "bntt_axpy": [_P, _P, _P, _PU, _SZ, _PU, _PU],
     "bntt_ntt_fwd": [_P, _P, _P, ctypes.c_int, _PU, _PU],
+    "bntt_batch_plan_create": [ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(NttTask), _SZ,
+                               ctypes.POINTER(MulTask), _SZ, _PU, _PU, ctypes.POINTER(TuneEntry), _SZ,
+                               ctypes.c_int],
+    "bntt_batch_plan_execute": [_P, _P],
+    "bntt_batch_plan_info": [_P, ctypes.POINTER(BatchInfo)],
+    "bntt_batch_plan_destroy": [_P],
+    "bntt_ntt_launch_each_dev": [ctypes.POINTER(NttTask), _SZ, _PU, _PU, _P, _P],
 }
 # entry points that do not return a status
 _OTHER = {

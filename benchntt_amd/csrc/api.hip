@@ -14,6 +14,7 @@
 #include <string>
 
 #include "../../include/bntt.h"
+#include "internal.hpp"
 #include "kernels.hpp"
 
 using bntt::Mod;
@@ -196,6 +197,11 @@ int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, cons
 }
 
 }  // namespace
+
+int bntt_set_error(int code, const char *msg) { return fail(code, "%s", msg); }
+int bntt_set_hip_error(hipError_t e, const char *what) { return hip_fail(e, what); }
+int bntt_ensure_init() { return ensure_init(); }
+bntt::Mod bntt_make_mod(const bntt_u128 *m, const bntt_u128 *mu) { return make_mod(m, mu); }
 
 #pragma GCC visibility push(default)
 extern "C" {

@@ -1,0 +1,642 @@
+// batch.hip -- uneven batches of forward NTTs (and pointwise products of their
+// outputs) in one persistent work-queue kernel, replayed from a hipGraph.
+//
+// A plan turns a list of transforms of sizes 2^1..2^24, each with its own
+// input, output and twiddle table, plus a list of pointwise products
+// c = a * b mod q, into "levels" of equal-size work items:
+//   - a single-pass item runs all stages of up to 2048/N whole transforms
+//     (N <= 2048) that share one twiddle table;
+//   - a multi-pass transform (larger N) contributes one item per 2048-value
+//     block to each of its ceil(log2 N / 8) passes, pass p at level p, through
+//     a plan-owned intermediate buffer;
+//   - a product contributes one item per 2048 elements at the level after the
+//     last pass of the transforms whose outputs it reads.
+// Each level is one launch of the same persistent kernel: a grid sized to the
+// resident capacity of the chip whose workgroups take items from an atomic
+// ticket until the level's list is exhausted, the last workgroup resetting the
+// counters for the next launch.  Items are ordered largest-first with the
+// memory-bound products spread among the compute-bound transform items.
+// Kernel boundaries order the levels, so no workgroup ever waits on another.
+//
+// The radix (points per thread per register sub-pass: 2, 4 or 8, hence the
+// LDS tile of 512/1024/2048 values of a 256-thread workgroup) and the number
+// of passes are chosen per transform size from a tuning table measured on the
+// device (benchntt_amd/autotune.py), or from built-in defaults.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/bntt.h"
+#include "internal.hpp"
+#include "ntt_core.hpp"
+
+namespace bntt {
+
+constexpr int kQueueThreads = 256;
+constexpr int kQueueMaxElog = 3;
+constexpr int kQueueTile = kQueueThreads << kQueueMaxElog;  // 2048 values
+// data tile plus at most as many staged twiddles: within the default 64 KiB
+constexpr size_t kQueueMaxLds = 2 * sizeof(U128) * kQueueTile;
+
+enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
+
+struct DevDesc {
+    const U128 *x;
+    U128 *y;
+    U128 *tmp;
+    const U128 *tw;
+};
+
+struct DevMul {
+    const U128 *a;
+    const U128 *b;
+    U128 *c;
+    uint64_t n;
+};
+
+struct WorkItem {
+    uint8_t kind, L, S, k;
+    uint8_t Glog, elog, pass, np;
+    uint32_t desc;    // first descriptor (NTT) or product index (MUL)
+    uint32_t nvalid;  // groups (NTT) or elements (MUL)
+    uint64_t first;   // first group of the pass (NTT multi-pass) or element offset (MUL)
+    uint64_t pad;
+};
+static_assert(sizeof(WorkItem) == 32, "work item layout");
+
+struct QueueCtl {
+    uint32_t ticket;
+    uint32_t done;
+    uint32_t pad[2];
+};
+
+// One transform item: load, stage twiddles, network, store.
+template <int ELOG>
+__device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, bool slow,
+                                                 const Mod &md)
+{
+    if (!slow)
+        run_group_network<ELOG, true, true>(lds, tv, k, S, mbase, md);
+    else
+        run_group_network<ELOG, false, true>(lds, tv, k, S, mbase, md);
+}
+
+__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkItem &it,
+                            const DevDesc *__restrict__ descs, const Mod &md)
+{
+    const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
+    const int G = 1 << Glog, E = 1 << elog;
+    const int mlog = L - k;
+    const int nvalid = (int)it.nvalid;
+    bool big = false;
+    const DevDesc d0 = descs[it.desc];
+    const U128 *src;
+    U128 *dst;
+    if (it.np == 1) {
+        src = d0.x;
+        dst = d0.y;
+    } else {
+        const int p = it.pass, np = it.np;
+        dst = ((np - 1 - p) % 2 == 0) ? d0.y : d0.tmp;
+        src = (p == 0) ? d0.x : (((np - p) % 2 == 0) ? d0.y : d0.tmp);
+    }
+    if (mlog == 0) {
+        // whole transforms, one per group, each with its own buffers
+        if ((int)threadIdx.x < nvalid) {
+            s_x[threadIdx.x] = descs[it.desc + threadIdx.x].x;
+            s_y[threadIdx.x] = descs[it.desc + threadIdx.x].y;
+        }
+        __syncthreads();
+    }
+    // HBM -> LDS
+    for (int j = 0; j < E; j++) {
+        const int e = threadIdx.x + j * kQueueThreads;
+        if (mlog == 0) {
+            const int g = e >> k, ti = e & ((1 << k) - 1);
+            if (g < nvalid) {
+                const U128 a = ld16(s_x[g] + ti);
+                big |= is_big(a);
+                lds[e] = a;
+            }
+        } else {
+            const int g = e & (G - 1), ti = e >> Glog;
+            if (g < nvalid) {
+                const size_t M = it.first + (size_t)g;
+                const U128 a = ld16(src + ((size_t)ti << mlog) + M);
+                big |= is_big(a);
+                lds[(g << k) + ti] = a;
+            }
+        }
+    }
+    const uint32_t mbase = (uint32_t)it.first;
+    const int wlog = Glog < S ? Glog : S;
+    const uint32_t W = 1u << wlog;
+    const uint32_t P0start = (wlog == Glog) ? (mbase & ((1u << S) - 1u)) : 0u;
+    R31 *tws = reinterpret_cast<R31 *>(lds + ((size_t)G << k));
+    big |= stage_twiddles(tws, d0.tw, S, k, W, P0start);
+    const bool slow = __syncthreads_or(big) || md.pm == 0;
+    TwView tv;
+    tv.lds = tws;
+    tv.glob = d0.tw;
+    tv.W = W;
+    switch (elog) {
+    case 1:
+        run_network_elog<1>(lds, tv, k, S, mbase, slow, md);
+        break;
+    case 2:
+        run_network_elog<2>(lds, tv, k, S, mbase, slow, md);
+        break;
+    default:
+        run_network_elog<3>(lds, tv, k, S, mbase, slow, md);
+        break;
+    }
+    // LDS -> HBM
+    for (int j = 0; j < E; j++) {
+        const int e = threadIdx.x + j * kQueueThreads;
+        if (mlog == 0) {
+            const int g = e >> k;
+            if (g < nvalid) st16(s_y[g] + (e & ((1 << k) - 1)), lds[e]);
+        } else {
+            if ((e >> k) < nvalid) st16(dst + (it.first << k) + e, lds[e]);
+        }
+    }
+}
+
+__device__ void do_mul_item(const WorkItem &it, const DevMul *__restrict__ muls, const Mod &md)
+{
+    const DevMul mu = muls[it.desc];
+#pragma unroll
+    for (int j = 0; j < kQueueTile / kQueueThreads; j++) {
+        const uint32_t e = threadIdx.x + j * kQueueThreads;
+        if (e < it.nvalid) {
+            const size_t i = it.first + e;
+            const U128 a = ld16(mu.a + i), b = ld16(mu.b + i);
+            U128 r;
+            if (md.pm && below_2_124(a, b))
+                r = mulmod_r31(a, to_r31(b), md.c);
+            else
+                r = mulmod_ref(a, b, md.m, md.mu);
+            st16(mu.c + i, r);
+        }
+    }
+}
+
+__global__ __launch_bounds__(kQueueThreads) void ntt_queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
+                                                                   const DevDesc *__restrict__ descs,
+                                                                   const DevMul *__restrict__ muls, QueueCtl *ctl,
+                                                                   Mod md)
+{
+    extern __shared__ U128 lds[];
+    __shared__ uint32_t s_ticket;
+    __shared__ const U128 *s_x[kQueueThreads];  // buffers of the transforms of a packed item
+    __shared__ U128 *s_y[kQueueThreads];
+    for (;;) {
+        if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->ticket, 1u);
+        __syncthreads();
+        const uint32_t t = s_ticket;
+        __syncthreads();
+        if (t >= nitems) break;
+        const WorkItem it = items[t];
+        if (it.kind == ITEM_NTT)
+            do_ntt_item(lds, s_x, s_y, it, descs, md);
+        else
+            do_mul_item(it, muls, md);
+    }
+    // every workgroup has drawn a ticket past the end; the last one to leave
+    // rearms the counters for the next launch of this level or the next level
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t d = atomicAdd(&ctl->done, 1u);
+        if (d == gridDim.x - 1) {
+            atomicExch(&ctl->ticket, 0u);
+            atomicExch(&ctl->done, 0u);
+        }
+    }
+}
+
+}  // namespace bntt
+
+// ------------------------------------------------------------------ host side
+
+using bntt::DevDesc;
+using bntt::DevMul;
+using bntt::Mod;
+using bntt::U128;
+using bntt::WorkItem;
+
+struct bntt_batch_plan {
+    int dev = -1;
+    Mod md{};
+    std::vector<size_t> level_items;     // items per level
+    std::vector<size_t> level_offset;    // first item of each level in items_d
+    std::vector<size_t> level_lds;       // dynamic LDS bytes of each level's launch
+    std::vector<int> level_grid;         // workgroups of each level's launch
+    WorkItem *items_d = nullptr;
+    DevDesc *descs_d = nullptr;
+    DevMul *muls_d = nullptr;
+    bntt::QueueCtl *ctl_d = nullptr;
+    U128 *tmp_d = nullptr;
+    size_t tmp_bytes = 0;
+    size_t nitems = 0;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::mutex mu;
+};
+
+namespace {
+
+struct TuneChoice {
+    int elog;
+    int np;
+};
+
+TuneChoice default_choice(int L)
+{
+    if (L <= 11) return {L < 3 ? L : 3, 1};
+    return {3, (L + 7) / 8};
+}
+
+bool valid_choice(int L, const TuneChoice &c)
+{
+    if (c.elog < 1 || c.elog > bntt::kQueueMaxElog || c.np < 1 || c.np > 3) return false;
+    const int tile_log = 8 + c.elog;
+    if (c.np == 1) return L <= tile_log && L >= c.elog;
+    // every pass must fit the tile and hold at least elog stages
+    for (int p = 0, S = 0; p < c.np; p++) {
+        const int k = (L - S) / (c.np - p);
+        if (k > tile_log || k < c.elog) return false;
+        S += k;
+    }
+    return true;
+}
+
+}  // namespace
+
+namespace {
+
+template <typename T>
+int upload(T **dst, const std::vector<T> &v)
+{
+    if (v.empty()) return BNTT_OK;
+    hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), v.size() * sizeof(T));
+    if (e != hipSuccess) return bntt_set_error(BNTT_ENOMEM, "hipMalloc for a batch plan failed");
+    e = hipMemcpy(*dst, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return bntt_set_hip_error(e, "hipMemcpy of a batch plan");
+    return BNTT_OK;
+}
+
+void free_plan(bntt_batch_plan *p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->items_d);
+    (void)hipFree(p->descs_d);
+    (void)hipFree(p->muls_d);
+    (void)hipFree(p->ctl_d);
+    (void)hipFree(p->tmp_d);
+    delete p;
+}
+
+size_t item_lds_bytes(int k, int S, int Glog) { return bntt::pass_lds_bytes(k, S, Glog); }
+
+int launch_levels(bntt_batch_plan *p, hipStream_t s)
+{
+    for (size_t l = 0; l < p->level_items.size(); l++) {
+        if (p->level_items[l] == 0) continue;
+        hipLaunchKernelGGL(bntt::ntt_queue_kernel, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
+                           p->level_lds[l], s, p->items_d + p->level_offset[l], (uint32_t)p->level_items[l],
+                           p->descs_d, p->muls_d, p->ctl_d, p->md);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return bntt_set_hip_error(e, "queue kernel launch");
+    }
+    return BNTT_OK;
+}
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, size_t ntasks,
+                           const bntt_mul_task *muls, size_t nmuls, const bntt_u128 *m, const bntt_u128 *mu,
+                           const bntt_tune_entry *tune, size_t ntune, int flags)
+{
+    int rc = bntt_ensure_init();
+    if (rc) return rc;
+    if (!out || !m || !mu) return bntt_set_error(BNTT_EINVAL, "null pointer argument");
+    *out = nullptr;
+    if ((ntasks && !tasks) || (nmuls && !muls)) return bntt_set_error(BNTT_EINVAL, "null task list");
+    if (ntasks > 0xffffffffu || nmuls > 0xffffffffu) return bntt_set_error(BNTT_EINVAL, "too many tasks");
+
+    TuneChoice choice[bntt::kMaxLog2 + 1];
+    for (int L = 0; L <= bntt::kMaxLog2; L++) choice[L] = default_choice(L < 1 ? 1 : L);
+    for (size_t i = 0; i < ntune; i++) {
+        const int L = tune[i].log2n;
+        if (L < 1 || L > bntt::kMaxLog2) return bntt_set_error(BNTT_EINVAL, "tuning entry log2n out of range");
+        const TuneChoice c{tune[i].elog, tune[i].passes};
+        if (!valid_choice(L, c)) return bntt_set_error(BNTT_EINVAL, "tuning entry not valid for its size");
+        choice[L] = c;
+    }
+
+    bntt_batch_plan *p = new bntt_batch_plan();
+    hipGetDevice(&p->dev);
+    p->md = bntt_make_mod(m, mu);
+    p->use_graph = (flags & BNTT_PLAN_GRAPH) != 0;
+
+    // descriptors: one per transform; multi-pass transforms get tmp space
+    std::vector<DevDesc> descs(ntasks);
+    std::vector<size_t> tmp_off(ntasks, 0);
+    size_t tmp_elems = 0;
+    for (size_t i = 0; i < ntasks; i++) {
+        const bntt_ntt_task &t = tasks[i];
+        const int L = t.log2n;
+        if (L < 1 || L > bntt::kMaxLog2) {
+            free_plan(p);
+            return bntt_set_error(BNTT_EINVAL, "task log2n out of range");
+        }
+        if (!t.x || !t.y || !t.tw || (reinterpret_cast<uintptr_t>(t.x) | reinterpret_cast<uintptr_t>(t.y) |
+                                      reinterpret_cast<uintptr_t>(t.tw)) & 15u) {
+            free_plan(p);
+            return bntt_set_error(BNTT_EINVAL, "task pointers must be non-null and 16-byte aligned");
+        }
+        descs[i].x = static_cast<const U128 *>(t.x);
+        descs[i].y = static_cast<U128 *>(t.y);
+        descs[i].tw = static_cast<const U128 *>(t.tw);
+        descs[i].tmp = nullptr;
+        if (choice[L].np > 1) {
+            const size_t n = (size_t)1 << L;
+            const char *xb = static_cast<const char *>(t.x), *yb = static_cast<const char *>(t.y);
+            if (yb < xb + n * 16 && xb < yb + n * 16) {
+                free_plan(p);
+                return bntt_set_error(BNTT_EINVAL, "multi-pass transforms need y not overlapping x");
+            }
+            tmp_off[i] = tmp_elems;
+            tmp_elems += n;
+        }
+    }
+    if (tmp_elems) {
+        p->tmp_bytes = tmp_elems * sizeof(U128);
+        if (hipMalloc(&p->tmp_d, p->tmp_bytes) != hipSuccess) {
+            free_plan(p);
+            return bntt_set_error(BNTT_ENOMEM, "hipMalloc for batch intermediates failed");
+        }
+        for (size_t i = 0; i < ntasks; i++)
+            if (choice[tasks[i].log2n].np > 1) descs[i].tmp = p->tmp_d + tmp_off[i];
+    }
+
+    // transform items, by level; single-pass transforms of one size and one
+    // twiddle table are packed several per item (descriptors made contiguous)
+    std::vector<std::vector<WorkItem>> ntt_lv(4), mul_lv(5);
+    std::vector<DevDesc> packed;
+    packed.reserve(ntasks);
+    std::vector<size_t> order(ntasks);
+    for (size_t i = 0; i < ntasks; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        if (tasks[a].log2n != tasks[b].log2n) return tasks[a].log2n > tasks[b].log2n;
+        return reinterpret_cast<uintptr_t>(tasks[a].tw) < reinterpret_cast<uintptr_t>(tasks[b].tw);
+    });
+    std::vector<int> out_level(ntasks, 0);  // level at which each transform's output is final
+    size_t i0 = 0;
+    while (i0 < ntasks) {
+        const size_t ti = order[i0];
+        const int L = tasks[ti].log2n;
+        const TuneChoice c = choice[L];
+        if (c.np == 1) {
+            const int Glog = 8 + c.elog - L;  // transforms per item
+            size_t i1 = i0;
+            while (i1 < ntasks && tasks[order[i1]].log2n == L && tasks[order[i1]].tw == tasks[ti].tw &&
+                   (i1 - i0) < ((size_t)1 << Glog))
+                i1++;
+            WorkItem w{};
+            w.kind = bntt::ITEM_NTT;
+            w.L = (uint8_t)L;
+            w.S = 0;
+            w.k = (uint8_t)L;
+            w.Glog = (uint8_t)Glog;
+            w.elog = (uint8_t)c.elog;
+            w.pass = 0;
+            w.np = 1;
+            w.desc = (uint32_t)packed.size();
+            w.nvalid = (uint32_t)(i1 - i0);
+            w.first = 0;
+            for (size_t j = i0; j < i1; j++) {
+                packed.push_back(descs[order[j]]);
+                out_level[order[j]] = 0;
+            }
+            ntt_lv[0].push_back(w);
+            i0 = i1;
+        } else {
+            const uint32_t di = (uint32_t)packed.size();
+            packed.push_back(descs[ti]);
+            int S = 0;
+            for (int pidx = 0; pidx < c.np; pidx++) {
+                const int k = (L - S) / (c.np - pidx);
+                const int Glog = 8 + c.elog - k;
+                const size_t groups = (size_t)1 << (L - k);
+                for (size_t g0 = 0; g0 < groups; g0 += ((size_t)1 << Glog)) {
+                    WorkItem w{};
+                    w.kind = bntt::ITEM_NTT;
+                    w.L = (uint8_t)L;
+                    w.S = (uint8_t)S;
+                    w.k = (uint8_t)k;
+                    w.Glog = (uint8_t)Glog;
+                    w.elog = (uint8_t)c.elog;
+                    w.pass = (uint8_t)pidx;
+                    w.np = (uint8_t)c.np;
+                    w.desc = di;
+                    w.nvalid = (uint32_t)std::min(groups - g0, (size_t)1 << Glog);
+                    w.first = g0;
+                    ntt_lv[pidx].push_back(w);
+                }
+                S += k;
+            }
+            out_level[ti] = c.np - 1;
+            i0++;
+        }
+    }
+
+    // products: level after the last pass of the transforms they read
+    std::unordered_map<const void *, int> level_of_output;
+    for (size_t j = 0; j < ntasks; j++) {
+        auto it = level_of_output.find(tasks[j].y);
+        const int lv = out_level[j] + 1;
+        if (it == level_of_output.end() || it->second < lv) level_of_output[tasks[j].y] = lv;
+    }
+    std::vector<DevMul> mv(nmuls);
+    for (size_t i = 0; i < nmuls; i++) {
+        const bntt_mul_task &t = muls[i];
+        if (!t.a || !t.b || !t.c ||
+            (reinterpret_cast<uintptr_t>(t.a) | reinterpret_cast<uintptr_t>(t.b) | reinterpret_cast<uintptr_t>(t.c)) &
+                15u) {
+            free_plan(p);
+            return bntt_set_error(BNTT_EINVAL, "product pointers must be non-null and 16-byte aligned");
+        }
+        mv[i] = DevMul{static_cast<const U128 *>(t.a), static_cast<const U128 *>(t.b), static_cast<U128 *>(t.c),
+                       t.n};
+        int lvl = 0;
+        for (const void *src : {t.a, t.b}) {
+            const auto it = level_of_output.find(src);
+            if (it != level_of_output.end()) lvl = std::max(lvl, it->second);
+        }
+        for (uint64_t off = 0; off < t.n; off += bntt::kQueueTile) {
+            WorkItem w{};
+            w.kind = bntt::ITEM_MUL;
+            w.desc = (uint32_t)i;
+            w.nvalid = (uint32_t)std::min<uint64_t>(t.n - off, bntt::kQueueTile);
+            w.first = off;
+            mul_lv[lvl].push_back(w);
+        }
+    }
+
+    // order each level: largest transform items first, products spread evenly
+    std::vector<WorkItem> all;
+    int grid_cap = 0;
+    {
+        hipDeviceProp_t prop;
+        hipGetDeviceProperties(&prop, p->dev);
+        grid_cap = prop.multiProcessorCount;
+    }
+    const size_t nlevels = 5;
+    for (size_t l = 0; l < nlevels; l++) {
+        std::vector<WorkItem> nt = l < ntt_lv.size() ? ntt_lv[l] : std::vector<WorkItem>();
+        std::stable_sort(nt.begin(), nt.end(), [](const WorkItem &x, const WorkItem &y) {
+            const int cx = (int)x.k << x.Glog, cy = (int)y.k << y.Glog;  // stages x values ~ work
+            return cx > cy;
+        });
+        const std::vector<WorkItem> &mm = mul_lv[l];
+        std::vector<WorkItem> lv;
+        lv.reserve(nt.size() + mm.size());
+        size_t a_i = 0, b_i = 0;
+        while (a_i < nt.size() || b_i < mm.size()) {
+            // keep the fraction of products emitted equal to their share of the level
+            if (b_i < mm.size() && (a_i >= nt.size() || b_i * nt.size() <= a_i * mm.size()))
+                lv.push_back(mm[b_i++]);
+            else
+                lv.push_back(nt[a_i++]);
+        }
+        size_t lds = 0;
+        for (const WorkItem &w : lv)
+            if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, item_lds_bytes(w.k, w.S, w.Glog));
+        int per_cu = 0;
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::ntt_queue_kernel, bntt::kQueueThreads, lds);
+        if (per_cu < 1) per_cu = 1;
+        const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), (size_t)grid_cap * per_cu);
+        p->level_offset.push_back(all.size());
+        p->level_items.push_back(lv.size());
+        p->level_lds.push_back(lds);
+        p->level_grid.push_back(grid);
+        all.insert(all.end(), lv.begin(), lv.end());
+    }
+    while (!p->level_items.empty() && p->level_items.back() == 0) {
+        p->level_items.pop_back();
+        p->level_offset.pop_back();
+        p->level_lds.pop_back();
+        p->level_grid.pop_back();
+    }
+    p->nitems = all.size();
+
+    for (size_t lds : p->level_lds)
+        if (lds > bntt::kQueueMaxLds) {
+            free_plan(p);
+            return bntt_set_error(BNTT_EINVAL, "internal: batch item exceeds the LDS budget");
+        }
+    if ((rc = upload(&p->items_d, all)) || (rc = upload(&p->descs_d, packed)) || (rc = upload(&p->muls_d, mv))) {
+        free_plan(p);
+        return rc;
+    }
+    if (hipMalloc(&p->ctl_d, sizeof(bntt::QueueCtl)) != hipSuccess) {
+        free_plan(p);
+        return bntt_set_error(BNTT_ENOMEM, "hipMalloc for queue counters failed");
+    }
+    if (hipMemset(p->ctl_d, 0, sizeof(bntt::QueueCtl)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        free_plan(p);
+        return bntt_set_error(BNTT_EHIP, "initialising queue counters failed");
+    }
+    *out = p;
+    return BNTT_OK;
+}
+
+int bntt_batch_plan_execute(bntt_batch_plan *p, void *stream)
+{
+    if (!p) return bntt_set_error(BNTT_EINVAL, "null plan");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(p->mu);
+    if (!p->use_graph || s == nullptr) return launch_levels(p, s);
+    if (!p->exec) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamBeginCapture");
+        const int rc = launch_levels(p, s);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGraphDestroy(g);
+            return bntt_set_hip_error(e, "hipGraphInstantiate");
+        }
+        p->graph = g;
+    }
+    const hipError_t e = hipGraphLaunch(p->exec, s);
+    if (e != hipSuccess) return bntt_set_hip_error(e, "hipGraphLaunch");
+    return BNTT_OK;
+}
+
+int bntt_batch_plan_info(const bntt_batch_plan *p, bntt_batch_info *info)
+{
+    if (!p || !info) return bntt_set_error(BNTT_EINVAL, "null argument");
+    std::memset(info, 0, sizeof *info);
+    info->levels = (int)p->level_items.size();
+    info->items = p->nitems;
+    info->tmp_bytes = p->tmp_bytes;
+    for (size_t l = 0; l < p->level_items.size() && l < 8; l++) {
+        info->level_items[l] = p->level_items[l];
+        info->level_grid[l] = p->level_grid[l];
+        info->level_lds_bytes[l] = (uint32_t)p->level_lds[l];
+    }
+    return BNTT_OK;
+}
+
+int bntt_batch_plan_destroy(bntt_batch_plan *p)
+{
+    if (!p) return BNTT_OK;
+    const hipError_t e = hipDeviceSynchronize();
+    free_plan(p);
+    if (e != hipSuccess) return bntt_set_hip_error(e, "hipDeviceSynchronize");
+    return BNTT_OK;
+}
+
+int bntt_ntt_launch_each_dev(const bntt_ntt_task *tasks, size_t ntasks, const bntt_u128 *m, const bntt_u128 *mu,
+                             void *tmp, void *stream)
+{
+    int rc = bntt_ensure_init();
+    if (rc) return rc;
+    if (ntasks && !tasks) return bntt_set_error(BNTT_EINVAL, "null task list");
+    if (!m || !mu) return bntt_set_error(BNTT_EINVAL, "null modulus");
+    const Mod md = bntt_make_mod(m, mu);
+    for (size_t i = 0; i < ntasks; i++) {
+        const bntt_ntt_task &t = tasks[i];
+        if (t.log2n < bntt::kMinLog2 || t.log2n > bntt::kMaxLog2)
+            return bntt_set_error(BNTT_EINVAL, "task log2n out of range");
+        if (t.log2n > bntt::kLdsMaxLog2 && !tmp)
+            return bntt_set_error(BNTT_EINVAL, "tasks above 2^12 points need tmp");
+        const hipError_t e = bntt::launch_ntt_uniform(static_cast<U128 *>(t.y), static_cast<const U128 *>(t.x),
+                                                      static_cast<const U128 *>(t.tw), t.log2n, 1, md,
+                                                      static_cast<U128 *>(tmp), static_cast<hipStream_t>(stream));
+        if (e != hipSuccess) return bntt_set_hip_error(e, "ntt kernel launch");
+    }
+    return BNTT_OK;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

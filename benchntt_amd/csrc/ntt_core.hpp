@@ -1,0 +1,198 @@
+// ntt_core.hpp -- device building blocks of the forward constant-geometry NTT,
+// shared by the launch-per-call kernel (ntt.hip) and the persistent
+// work-queue kernel (batch.hip).
+//
+// Reference network (ntt_scalar.c:582-627): log2(N) stages; at stage s
+// (t = 2^s), for j in [0, N/2):
+//     p = mulmod(tw[t + (j mod t)], in[j + N/2])
+//     out[2j] = addmod(in[j], p),  out[2j+1] = submod(in[j], p).
+// Each stage pairs the elements whose indices differ in the top bit and writes
+// them to positions that differ in the bottom bit: the index rotates left by
+// one.  Hence any k consecutive stages starting at stage S act independently
+// on the 2^(L-k) groups of 2^k elements that share their low L-k index bits M
+// (inputs at stride 2^(L-k)), and each group's outputs land contiguously at
+// M*2^k.  Inside a group, local stage r is again a constant-geometry stage of
+// size 2^k whose butterfly i uses
+//     tw[2^(S+r) + ((M mod 2^S) << r) + (i mod 2^r)].
+// The same split applies recursively inside a group: a workgroup holds G
+// groups in LDS and runs register sub-passes of up to 2^ELOG points per
+// thread (LDS -> VGPR -> LDS) until all k stages are done.
+#pragma once
+
+#include "kernels.hpp"
+
+namespace bntt {
+
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ U128 ld16(const U128 *p)
+{
+    const v4u t = *reinterpret_cast<const v4u *>(p);
+    U128 r;
+    r.v[0] = t.x;
+    r.v[1] = t.y;
+    r.v[2] = t.z;
+    r.v[3] = t.w;
+    return r;
+}
+
+__device__ __forceinline__ void st16(U128 *p, const U128 &a)
+{
+    v4u t;
+    t.x = a.v[0];
+    t.y = a.v[1];
+    t.z = a.v[2];
+    t.w = a.v[3];
+    *reinterpret_cast<v4u *>(p) = t;
+}
+
+constexpr int kMaxThreads = 512;
+// Dynamic LDS of the largest launch: a 4096-point transform and its 4095
+// staged twiddles (the CU has 160 KiB).
+constexpr size_t kMaxLdsBytes = 128 * 1024;
+
+// A value is "small" when its top limb is below that of q = 2^124 - c for
+// every c < 2^16 (0x0FFFFFFF): then it is below q.  When every input and
+// twiddle of a workgroup is small, every value the network produces is at
+// most q (< 2^124), so the fast exact reduction is valid for all its products
+// and equals the reference's Barrett result; otherwise the workgroup runs the
+// literal reference formulas.
+__device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFFFFFu; }
+
+// Twiddle lookup of one register sub-pass.  Stage table r holds, for the W
+// distinct group prefixes of this workgroup, the entries
+// tw[2^(S+r) + ((P0_start + rel) << r) + j], rel < W, j < 2^r, at LDS offset
+// W*(2^r - 1) + (rel << r) + j.
+struct TwView {
+    const R31 *lds;     // staged digit tables (fast path)
+    const U128 *glob;   // the caller's table (literal path)
+    uint32_t W;         // distinct prefixes staged
+};
+
+// KP stages of the constant-geometry network on 2^KP values held in registers.
+// Local stages r0 .. r0+KP-1 of a group network that starts at global stage S;
+// rel: the group's staged-prefix index; P0: the group's prefix (low S bits of
+// its M); Mlow: low r0 bits of the sub-group index.
+template <int KP, bool FAST, bool PM>
+__device__ __forceinline__ void reg_network(U128 (&v)[1 << KP], const TwView &tv, int S, int r0, uint32_t rel,
+                                            uint32_t P0, uint32_t Mlow, const Mod &md)
+{
+    constexpr int H = 1 << (KP - 1);
+#pragma unroll
+    for (int rp = 0; rp < KP; rp++) {
+        const int r = r0 + rp;
+        U128 w[1 << KP];
+        if constexpr (FAST) {
+            const R31 *tb = tv.lds + tv.W * ((1u << r) - 1u) + (rel << r) + (Mlow << rp);
+            R31 t[H];
+#pragma unroll
+            for (int j = 0; j < (1 << rp) && j < H; j++) t[j] = tb[j];
+#pragma unroll
+            for (int i = 0; i < H; i++) {
+                const U128 p = mulmod_r31(v[i + H], t[i & ((1 << rp) - 1)], md.c);
+                w[2 * i] = addmod(v[i], p, md);
+                w[2 * i + 1] = submod(v[i], p, md);
+            }
+        } else {
+            const uint32_t P = (P0 << r0) | Mlow;
+            const U128 *tb = tv.glob + (1u << (S + r)) + (P << rp);
+            U128 t[H];
+#pragma unroll
+            for (int j = 0; j < (1 << rp) && j < H; j++) t[j] = ld16(tb + j);
+#pragma unroll
+            for (int i = 0; i < H; i++) {
+                const U128 p = mulmod<PM>(t[i & ((1 << rp) - 1)], v[i + H], md);
+                w[2 * i] = addmod(v[i], p, md);
+                w[2 * i + 1] = submod(v[i], p, md);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * H; i++) v[i] = w[i];
+    }
+}
+
+// One register sub-pass of KP stages over the G groups of 2^k values in LDS.
+// Each thread owns 2^(ELOG-KP) sub-groups.
+template <int ELOG, int KP, bool FAST, bool PM>
+__device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, int S, int r0, uint32_t mbase,
+                                            const Mod &md)
+{
+    constexpr int Q = 1 << (ELOG - KP);
+    constexpr int R = 1 << KP;
+    const int T = blockDim.x;
+    const int sub_log = k - KP;  // sub-groups per group = 2^sub_log
+    U128 v[Q][R];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int sg = threadIdx.x + q * T;
+        const int g = sg >> sub_log;
+        const int Mp = sg & ((1 << sub_log) - 1);
+        const U128 *src = lds + (g << k) + Mp;
+#pragma unroll
+        for (int j = 0; j < R; j++) v[q][j] = src[j << sub_log];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        const int sg = threadIdx.x + q * T;
+        const int g = sg >> sub_log;
+        const int Mp = sg & ((1 << sub_log) - 1);
+        const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);
+        const uint32_t Mlow = (uint32_t)Mp & ((1u << r0) - 1u);
+        const uint32_t rel = (uint32_t)g & (tv.W - 1u);
+        reg_network<KP, FAST, PM>(v[q], tv, S, r0, rel, P0, Mlow, md);
+        U128 *dst = lds + (g << k) + (Mp << KP);
+#pragma unroll
+        for (int j = 0; j < R; j++) dst[j] = v[q][j];
+    }
+    __syncthreads();
+}
+
+template <int ELOG, bool FAST, bool PM>
+__device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase,
+                                                  const Mod &md)
+{
+    int r0 = 0;
+    while (r0 < k) {
+        const int kp = (k - r0) < ELOG ? (k - r0) : ELOG;
+        if (kp == ELOG) {
+            lds_subpass<ELOG, ELOG, FAST, PM>(lds, tv, k, S, r0, mbase, md);
+        } else if constexpr (ELOG >= 2) {
+            if (kp == 1) {
+                lds_subpass<ELOG, 1, FAST, PM>(lds, tv, k, S, r0, mbase, md);
+            } else if constexpr (ELOG >= 3) {
+                if (kp == 2) lds_subpass<ELOG, 2, FAST, PM>(lds, tv, k, S, r0, mbase, md);
+            }
+        }
+        r0 += kp;
+    }
+}
+
+// Stage, as digits, the twiddles that stages [S, S+k) use for W consecutive
+// group prefixes starting at P0start: stage table r (2^r entries per prefix)
+// sits at LDS offset W*(2^r - 1).  Returns whether any entry is not small.
+__device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict__ tw, int S, int k, uint32_t W,
+                                               uint32_t P0start)
+{
+    bool big = false;
+    const int ntw = (int)(W * ((1u << k) - 1u));
+    for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
+        // e = W*(2^r - 1) + rel*2^r + j  ->  stage r, offset within the stage table
+        const uint32_t u = (uint32_t)e / W + 1u;  // in [2^r, 2^(r+1))
+        const int r = 31 - __builtin_clz(u);
+        const uint32_t off = (uint32_t)e - W * ((1u << r) - 1u);
+        const U128 a = ld16(tw + (1u << (S + r)) + (P0start << r) + off);
+        big |= is_big(a);
+        tws[e] = to_r31(a);
+    }
+    return big;
+}
+
+// LDS bytes of one workgroup: the data, then the staged digit tables.
+__host__ __device__ inline size_t pass_lds_bytes(int k, int S, int Glog)
+{
+    const int wlog = Glog < S ? Glog : S;
+    return (sizeof(U128) << (Glog + k)) + (sizeof(R31) << wlog) * (((size_t)1 << k) - 1);
+}
+
+}  // namespace bntt

@@ -205,6 +205,99 @@ BNTT_HD U128 mulmod_pm(const U128 &a, const U128 &b, uint32_t c)
     return r;
 }
 
+// ------------------------------------------------- radix-2^31 exact mulmod
+//
+// On gfx950 v_mad_u64_u32 (32x32 -> 64 plus a 64-bit addend) issues at the
+// same rate as a 32-bit add, so the cost of a 128-bit product is its
+// instruction count, and in radix 2^32 most of that is carry handling.  With
+// the residue split into four 31-bit digits (124 = 4 * 31) every partial
+// product is below 2^62 and a column of four of them plus the incoming carry
+// stays below 2^64, so each column is a plain chain of multiply-adds with no
+// carry fixups: 16 v_mad_u64_u32 plus one mask and one 64-bit shift per column.
+
+struct R31 {
+    uint32_t d[4];  // value = d0 + d1*2^31 + d2*2^62 + d3*2^93, each d < 2^31
+};
+
+constexpr uint32_t kM31 = 0x7FFFFFFFu;
+
+// 32-bit limbs -> 31-bit digits, for a < 2^124.
+BNTT_HD R31 to_r31(const U128 &a)
+{
+    R31 r;
+    r.d[0] = a.v[0] & kM31;
+    r.d[1] = funnel(a.v[1], a.v[0], 31) & kM31;
+    r.d[2] = funnel(a.v[2], a.v[1], 30) & kM31;
+    r.d[3] = funnel(a.v[3], a.v[2], 29);
+    return r;
+}
+
+// a*w mod q for q = 2^124 - c (c < 2^16) and a, w < 2^124; w given in digits.
+// Canonical result in 32-bit limbs.
+//   x = a*w = sum_k d_k 2^(31k), k < 8      (columns normalized on the fly)
+//   V = sum_{k<4} (d_k + c*d_{k+4}) 2^(31k) = f + t*2^124,  t <= c
+//   W = f + c*t < 2^124 + 2^32,  result = W >= q ? W - q : W,
+//   with W >= q  <=>  W + c >= 2^124.
+BNTT_HD U128 mulmod_r31(const U128 &a_, const R31 &w, uint32_t c)
+{
+    const R31 a = to_r31(a_);
+    uint64_t t;
+    uint32_t d[8];
+    t = (uint64_t)a.d[0] * w.d[0];
+    d[0] = (uint32_t)t & kM31;
+    t = mad32(a.d[0], w.d[1], t >> 31);
+    t = mad32(a.d[1], w.d[0], t);
+    d[1] = (uint32_t)t & kM31;
+    t = mad32(a.d[0], w.d[2], t >> 31);
+    t = mad32(a.d[1], w.d[1], t);
+    t = mad32(a.d[2], w.d[0], t);
+    d[2] = (uint32_t)t & kM31;
+    t = mad32(a.d[0], w.d[3], t >> 31);
+    t = mad32(a.d[1], w.d[2], t);
+    t = mad32(a.d[2], w.d[1], t);
+    t = mad32(a.d[3], w.d[0], t);
+    d[3] = (uint32_t)t & kM31;
+    t = mad32(a.d[1], w.d[3], t >> 31);
+    t = mad32(a.d[2], w.d[2], t);
+    t = mad32(a.d[3], w.d[1], t);
+    d[4] = (uint32_t)t & kM31;
+    t = mad32(a.d[2], w.d[3], t >> 31);
+    t = mad32(a.d[3], w.d[2], t);
+    d[5] = (uint32_t)t & kM31;
+    t = mad32(a.d[3], w.d[3], t >> 31);
+    d[6] = (uint32_t)t & kM31;
+    d[7] = (uint32_t)(t >> 31);
+    // fold 2^124 = c (mod q): digits k and k+4 together, carries below 2^18
+    uint32_t f[4];
+    t = mad32(c, d[4], d[0]);
+    f[0] = (uint32_t)t & kM31;
+    t = mad32(c, d[5], (t >> 31) + d[1]);
+    f[1] = (uint32_t)t & kM31;
+    t = mad32(c, d[6], (t >> 31) + d[2]);
+    f[2] = (uint32_t)t & kM31;
+    t = mad32(c, d[7], (t >> 31) + d[3]);
+    f[3] = (uint32_t)t & kM31;
+    const uint32_t top = (uint32_t)(t >> 31);  // t <= c
+    // back to 32-bit limbs (f < 2^124), then add c*top
+    uint32_t k, w0, w1, w2, w3;
+    w0 = addc(f[0] | (f[1] << 31), c * top, 0, k);
+    w1 = addc((f[1] >> 1) | (f[2] << 30), 0, k, k);
+    w2 = addc((f[2] >> 2) | (f[3] << 29), 0, k, k);
+    w3 = (f[3] >> 3) + k;  // <= 2^28
+    uint32_t z0, z1, z2, z3;
+    z0 = addc(w0, c, 0, k);
+    z1 = addc(w1, 0, k, k);
+    z2 = addc(w2, 0, k, k);
+    z3 = w3 + k;
+    const bool ge = (z3 >> 28) != 0;
+    U128 r;
+    r.v[0] = ge ? z0 : w0;
+    r.v[1] = ge ? z1 : w1;
+    r.v[2] = ge ? z2 : w2;
+    r.v[3] = ge ? (z3 & 0x0FFFFFFFu) : w3;
+    return r;
+}
+
 // ---------------------------------------------------- literal reference Barrett
 
 struct W128 {

@@ -89,6 +89,75 @@ int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t c
                      const bntt_u128 *mu, void *tmp, void *stream);
 size_t bntt_ntt_tmp_bytes(int log2n, size_t count);
 
+/* ------------------------------------------------------------ uneven batches */
+/*
+ * No counterpart in the reference, whose ntt() handles one transform of a
+ * compile-time size (ntt_scalar.c:582-627): these entry points run many
+ * transforms of different sizes, each exactly the reference network.
+ */
+
+/* One forward transform of a batch: device pointers, 16-byte aligned. */
+typedef struct bntt_ntt_task {
+    const void *x;  /* 2^log2n residues in */
+    void *y;        /* 2^log2n residues out (must not overlap x above 2^11 points) */
+    const void *tw; /* 2^log2n-entry twiddle table */
+    int32_t log2n;  /* 1 .. 24 */
+    int32_t reserved;
+} bntt_ntt_task;
+
+/* One pointwise product c[i] = a[i] * b[i] mod m, i < n (device pointers).
+ * When a or b is the y of a transform in the same plan, the product runs
+ * after that transform. */
+typedef struct bntt_mul_task {
+    const void *a;
+    const void *b;
+    void *c;
+    uint64_t n;
+} bntt_mul_task;
+
+/* Per-size choice of the batch kernel: points per thread per register
+ * sub-pass 2^elog (1..3) and number of passes through device memory. */
+typedef struct bntt_tune_entry {
+    int32_t log2n;
+    int32_t elog;
+    int32_t passes;
+    int32_t reserved;
+} bntt_tune_entry;
+
+typedef struct bntt_batch_info {
+    int32_t levels;              /* launches per execution */
+    int32_t reserved;
+    uint64_t items;              /* work items over all levels */
+    uint64_t tmp_bytes;          /* plan-owned intermediate storage */
+    uint64_t level_items[8];
+    int32_t level_grid[8];       /* persistent workgroups per launch */
+    uint32_t level_lds_bytes[8]; /* dynamic LDS per workgroup */
+} bntt_batch_info;
+
+typedef struct bntt_batch_plan bntt_batch_plan;
+
+#define BNTT_PLAN_GRAPH 1 /* capture the launches in a hipGraph on the first
+                             execute with a non-null stream, replay after */
+
+/* Build a plan: device work lists, intermediates for multi-pass sizes, and
+ * per-size kernel choices (tune may be NULL for built-in defaults).  The plan
+ * records pointers, not data: execute it any number of times after the
+ * inputs change in place. */
+int bntt_batch_plan_create(bntt_batch_plan **plan, const bntt_ntt_task *tasks, size_t ntasks,
+                           const bntt_mul_task *muls, size_t nmuls, const bntt_u128 *m, const bntt_u128 *mu,
+                           const bntt_tune_entry *tune, size_t ntune, int flags);
+/* Enqueue one execution on `stream` (no synchronization). */
+int bntt_batch_plan_execute(bntt_batch_plan *plan, void *stream);
+int bntt_batch_plan_info(const bntt_batch_plan *plan, bntt_batch_info *info);
+/* Synchronizes the device, then frees the plan. */
+int bntt_batch_plan_destroy(bntt_batch_plan *plan);
+
+/* The same transforms with one launch sequence per transform, the plain path
+ * of bntt_ntt_fwd_dev applied in turn (tmp: 2^max_log2n residues when any
+ * task is above 2^12 points). */
+int bntt_ntt_launch_each_dev(const bntt_ntt_task *tasks, size_t ntasks, const bntt_u128 *m, const bntt_u128 *mu,
+                             void *tmp, void *stream);
+
 /* -------------------------------------------------------------- host pointers */
 
 int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m);

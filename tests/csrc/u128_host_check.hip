@@ -10,7 +10,8 @@ using bntt::U128;
 
 extern "C" {
 
-// op: 0 addmod_ref, 1 submod_ref, 2 mulmod_ref, 3 mulmod_pm, 4 mulmod<true>, 5 mulmod<false>
+// op: 0 addmod_ref, 1 submod_ref, 2 mulmod_ref, 3 mulmod_pm, 4 mulmod<true>, 5 mulmod<false>,
+//     6 mulmod_r31 (b converted to digits first)
 void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U128 *m, const U128 *mu, uint32_t c)
 {
     bntt::Mod md;
@@ -26,7 +27,8 @@ void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U12
         case 2: out[i] = bntt::mulmod_ref(a[i], b[i], md.m, md.mu); break;
         case 3: out[i] = bntt::mulmod_pm(a[i], b[i], c); break;
         case 4: out[i] = bntt::mulmod<true>(a[i], b[i], md); break;
-        default: out[i] = bntt::mulmod<false>(a[i], b[i], md); break;
+        case 5: out[i] = bntt::mulmod<false>(a[i], b[i], md); break;
+        default: out[i] = bntt::mulmod_r31(a[i], bntt::to_r31(b[i]), c); break;
         }
     }
 }

@@ -1,0 +1,171 @@
+"""GPU parity of uneven batches (persistent work-queue kernel, graph replay)
+against the oracle, transform by transform and product by product."""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+from benchntt_amd import config as C
+from benchntt_amd.residues import uniform
+from benchntt_amd.workloads import uneven_pairs
+from conftest import from_dev, to_dev
+
+pytestmark = pytest.mark.gpu
+
+QN, MUN = C.Q_NTT, C.MU_NTT
+
+
+def build(torch, wl):
+    dx = to_dev(torch, wl.x)
+    dy = torch.empty_like(dx)
+    dc = torch.empty((wl.pair_total, 2), dtype=torch.int64, device="cuda")
+    tabs = {l: to_dev(torch, t) for l, t in wl.tables.items()}
+    tasks, muls = [], []
+    for i, (l, off) in enumerate(zip(wl.log2n, wl.offsets)):
+        n = 1 << int(l)
+        tasks.append((dx[off:off + n], dy[off:off + n], tabs[int(l)]))
+    for j, poff in enumerate(wl.pair_offsets):
+        n = 1 << int(wl.log2n[2 * j])
+        a0, b0 = wl.offsets[2 * j], wl.offsets[2 * j + 1]
+        muls.append((dy[a0:a0 + n], dy[b0:b0 + n], dc[poff:poff + n]))
+    return dx, dy, dc, tabs, tasks, muls
+
+
+def check_all(orc, wl, y, c, threads=16, every=1):
+    """Every transform (or every `every`-th pair) and its product vs the oracle."""
+    def one(j):
+        outs = []
+        for i in (2 * j, 2 * j + 1):
+            l, off = int(wl.log2n[i]), int(wl.offsets[i])
+            n = 1 << l
+            want = orc.ntt(wl.x[off:off + n], wl.tables[l], QN, MUN)
+            if not np.array_equal(y[off:off + n], want):
+                return f"transform {i} (N=2^{l}) differs"
+            outs.append(want)
+        poff = int(wl.pair_offsets[j])
+        n = outs[0].shape[0]
+        if not np.array_equal(c[poff:poff + n], orc.vvmul(outs[0], outs[1], QN, MUN)):
+            return f"product {j} differs"
+        return None
+
+    with ThreadPoolExecutor(threads) as ex:
+        errs = [e for e in ex.map(one, range(0, len(wl.pair_offsets), every)) if e]
+    assert not errs, errs[:5]
+
+
+def test_small_uneven_batch(gpu, orc):
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(80, QN, seed=11, lo=1, hi=16)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc))
+    info = plan.info()
+    assert info["levels"] == 3 and info["items"] > 0
+    plan.close()
+
+
+def test_graph_replays_follow_inputs(gpu, orc):
+    """Replays of the captured graph, with eager work and a refill of the
+    inputs in between, each match the oracle (counters rearm every launch)."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(40, QN, seed=5, lo=4, hi=14)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=True)
+    stream = gpu.cuda.Stream()
+    for rep in range(3):
+        if rep:
+            wl.x = uniform(wl.total, QN, 100 + rep)
+            dx.copy_(to_dev(gpu, wl.x))
+            gpu.cuda.synchronize()
+        with gpu.cuda.stream(stream):
+            plan.execute()
+        stream.synchronize()
+        check_all(orc, wl, from_dev(dy), from_dev(dc))
+    plan.close()
+
+
+@pytest.mark.parametrize("tune", [
+    {l: (1, 1) for l in range(1, 10)} | {l: (2, 2) for l in range(10, 15)},
+    {l: (2, 1) for l in range(2, 11)} | {l: (3, 3) for l in range(11, 17)},
+    {l: (3, 2) for l in range(6, 17)},
+])
+def test_tuning_choices_agree(gpu, orc, tune):
+    """Every radix / pass-count choice gives the reference's outputs."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(60, QN, seed=7, lo=1, hi=16)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, tune=tune, graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc))
+    plan.close()
+
+
+def test_large_sizes_three_passes(gpu, orc):
+    from benchntt_amd.batch import BatchPlan
+
+    tasks, xs, tws, ys = [], [], [], []
+    for l in (17, 18, 20):
+        x, tw = uniform(1 << l, QN, l), uniform(1 << l, QN, 50 + l)
+        dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+        dy = gpu.empty_like(dx)
+        tasks.append((dx, dy, dtw))
+        xs.append(x), tws.append(tw), ys.append(dy)
+    plan = BatchPlan(tasks, [], QN, MUN, graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    for x, tw, dy in zip(xs, tws, ys):
+        assert np.array_equal(from_dev(dy), orc.ntt(x, tw, QN, MUN))
+    plan.close()
+
+
+def test_edge_values_in_batch(gpu, orc):
+    """Transforms holding values >= q (and >= 2^124) take the literal path
+    inside the batch kernel without disturbing their neighbours."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(24, QN, seed=3, lo=4, hi=13)
+    wl.x[5::97, 1] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    wl.x[7::131] = [QN & (2**64 - 1), QN >> 64]
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc))
+    plan.close()
+
+
+def test_launch_each_matches(gpu, orc):
+    from benchntt_amd.batch import launch_each
+
+    wl = uneven_pairs(30, QN, seed=9, lo=4, hi=15)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    tmp = gpu.empty((1 << 16, 2), dtype=gpu.int64, device="cuda")
+    launch_each(tasks, QN, MUN, tmp)
+    gpu.cuda.synchronize()
+    y = from_dev(dy)
+    for i, (l, off) in enumerate(zip(wl.log2n, wl.offsets)):
+        n = 1 << int(l)
+        assert np.array_equal(y[off:off + n], orc.ntt(wl.x[off:off + n], wl.tables[int(l)], QN, MUN)), i
+
+
+def test_uneven_20000_full_check(gpu, orc):
+    """The uneven-batch workload at full size: 20,000 transforms with N
+    log-uniform on 2^4..2^16 in pairs (about 2^27.6 coefficients), each pair's
+    product after its transforms; every transform and product bit-exact."""
+    from benchntt_amd.batch import BatchPlan, load_tune_table
+
+    wl = uneven_pairs(20000, QN, seed=2024)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, tune=load_tune_table(), graph=True)
+    stream = gpu.cuda.Stream()
+    with gpu.cuda.stream(stream):
+        plan.execute()
+    stream.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc), threads=16)
+    plan.close()

@@ -68,6 +68,7 @@ def test_device_arith_matches_oracle(chk, orc, q, mu):
     assert np.array_equal(run(chk, 5, a, b, q, mu), want)
     small = (a[:, 1] >> np.uint64(60) == 0) & (b[:, 1] >> np.uint64(60) == 0)
     assert np.array_equal(run(chk, 3, a[small], b[small], q, mu, c), want[small])
+    assert np.array_equal(run(chk, 6, a[small], b[small], q, mu, c), want[small])
 
 
 def test_device_arith_other_modulus(chk, orc):
