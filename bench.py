@@ -99,6 +99,53 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
             "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
 
 
+def uniform_rows(torch, np, bn, qn, mun, stream) -> dict:
+    """Plain transform (N=1024, one launch) and uniform batches: 8192 x N=1024
+    and N in {16..512} with 2^23 coefficients, one shared seeded twiddle table
+    per N, as one persistent work-queue execution (graph replay) against one
+    launch per transform from native code."""
+    from benchntt_amd.batch import BatchPlan, TaskList, launch_each, load_tune_table
+    from benchntt_amd.residues import uniform
+
+    dev = torch.device("cuda:0")
+    out = {}
+    tw = torch.from_numpy(uniform(1024, qn, 4).view(np.int64)).to(dev)
+    x1 = torch.from_numpy(uniform(1024, qn, 3).view(np.int64)).to(dev)
+    y1 = torch.empty_like(x1)
+    ms1 = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), 200, 20, stream)
+    out["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
+    out["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
+    tune = load_tune_table()
+    total = 1 << 23
+    xb = torch.from_numpy(uniform(total, qn, 5).view(np.int64)).to(dev)
+    yb = torch.empty_like(xb)
+    s2 = torch.cuda.Stream()
+    for L in (10, 4, 5, 6, 7, 8, 9):
+        n = 1 << L
+        cnt = total // n
+        twl = torch.from_numpy(uniform(n, qn, 40 + L).view(np.int64)).to(dev)
+        tasks = [(xb[i * n:(i + 1) * n], yb[i * n:(i + 1) * n], twl) for i in range(cnt)]
+        with torch.cuda.stream(s2):
+            plan = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True)
+            msp = time_kernel(torch, plan.execute, 20, 3, s2)
+            plan.close()
+            if L == 10:
+                msd = time_kernel(torch, lambda: bn.ntt(yb, xb, qn, mun, twl), 20, 3, s2)
+                tl = TaskList(tasks)
+                mse = time_kernel(torch, lambda: launch_each(tl, qn, mun), 2, 1, s2)
+        key = f"ntt_uniform_{cnt}x{n}"
+        out[f"{key}_NTTs_per_s"] = round(cnt / (msp * 1e-3), 1)
+        out[f"{key}_Gbutterflies_per_s"] = round(cnt * n * L / 2 / (msp * 1e-3) / 1e9, 2)
+        if L == 10:
+            out[f"{key}_single_launch_NTTs_per_s"] = round(cnt / (msd * 1e-3), 1)
+            out[f"{key}_per_launch_NTTs_per_s"] = round(cnt / (mse * 1e-3), 1)
+            out[f"{key}_persistent_vs_per_launch"] = round(mse / msp, 2)
+        log(f"uniform {cnt}x{n}: persistent {msp:.4f} ms ({out[f'{key}_NTTs_per_s']:.0f} NTT/s)"
+            + (f", one launch {msd:.4f} ms, per-transform launches {mse:.2f} ms" if L == 10 else ""))
+    log(f"ntt plain 1024: {ms1 * 1e3:.2f} us")
+    return out
+
+
 def uneven_batch(torch, np, bn, qn, mun, stream) -> dict:
     """20,000 forward NTTs, N log-uniform on 2^4..2^16 in pairs (seeded), one
     twiddle table per size, each pair's pointwise product after its two
@@ -155,8 +202,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "axpy"], default="all",
-                    help="'axpy' times only the headline kernel (for profiling runs)")
+    ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven"], default="all",
+                    help="time one workload only and print no JSON line (for profiling runs): "
+                         "axpy 2^24; copy (a 256 MiB device copy, the traffic calibration kernel); "
+                         "ntt-uniform (8192 x N=1024); uneven (20,000-transform batch)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -178,6 +227,21 @@ def main() -> int:
     props = torch.cuda.get_device_properties(0)
     log(f"device {props.name} ({getattr(props, 'gcnArchName', '?')}), {props.multi_processor_count} CUs")
 
+    if args.only in ("ntt-uniform", "uneven"):
+        qn, mun = C.Q_NTT, C.MU_NTT
+        if args.only == "uneven":
+            uneven_batch(torch, np, bn, qn, mun, stream)
+            return 0
+        uniform_rows(torch, np, bn, qn, mun, stream)
+        return 0
+    if args.only == "copy":
+        src = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+        src.fill_(1)
+        dst = torch.empty_like(src)
+        msc = time_kernel(torch, lambda: dst.copy_(src), args.steps, args.warmup, stream)
+        log(f"copy 256 MiB: {msc * 1e3:.1f} us, {2 * (1 << 28) / (msc * 1e-3) / 1e9:.1f} GB/s")
+        return 0
+
     n = 1 << 24
     q, mu, s = C.Q_BLAS, C.MU_BLAS, C.AXPY_SCALAR
     log("generating 3 x 2^24 seeded residues")
@@ -190,6 +254,8 @@ def main() -> int:
     ms_axpy = time_kernel(torch, lambda: bn.axpy(dc, da, db, s, q, mu), args.steps, args.warmup, stream)
     gbs = n * BYTES_PER_ELEM / (ms_axpy * 1e-3) / 1e9
     log(f"axpy 2^24: {ms_axpy * 1e3:.1f} us/call, {gbs:.1f} GB/s")
+    if args.only == "axpy":
+        return 0
 
     extras = {}
     if args.only == "all":
@@ -200,25 +266,8 @@ def main() -> int:
             extras[f"{name}_GBps"] = round(n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9, 1)
             log(f"{name} 2^24: {ms * 1e3:.1f} us/call, {extras[f'{name}_GBps']} GB/s")
 
-        # NTT rows over q_N = 2^124 - 719 with a seeded random twiddle table
-        qn, mun = C.Q_NTT, C.MU_NTT
-        tw = torch.from_numpy(uniform(1024, qn, 4).view(np.int64)).to(dev)
-        x1 = torch.from_numpy(uniform(1024, qn, 3).view(np.int64)).to(dev)
-        y1 = torch.empty_like(x1)
-        ms1 = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), max(args.steps, 200), args.warmup, stream)
-        extras["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
-        extras["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
-        cnt = 8192
-        xb = torch.from_numpy(uniform(cnt * 1024, qn, 5).view(np.int64)).to(dev)
-        yb = torch.empty_like(xb)
-        msb = time_kernel(torch, lambda: bn.ntt(yb, xb, qn, mun, tw), max(args.steps // 4, 20), 5, stream)
-        extras["ntt_uniform_8192x1024_NTTs_per_s"] = round(cnt / (msb * 1e-3), 1)
-        extras["ntt_uniform_8192x1024_Gbutterflies_per_s"] = round(cnt * 5120 / (msb * 1e-3) / 1e9, 2)
-        extras["ntt_uniform_vs_plain_launch"] = round((cnt / (msb * 1e-3)) / (1.0 / (ms1 * 1e-3)), 1)
-        log(f"ntt plain 1024: {ms1 * 1e3:.2f} us; uniform 8192x1024: {msb:.3f} ms "
-            f"({extras['ntt_uniform_8192x1024_NTTs_per_s']:.0f} NTT/s)")
-        del xb, yb
-        extras.update(uneven_batch(torch, np, bn, qn, mun, stream))
+        extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream))
+        extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream))
 
     traffic = load_traffic("axpy_2_24")
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

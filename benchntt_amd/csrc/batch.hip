@@ -40,8 +40,8 @@ namespace bntt {
 constexpr int kQueueThreads = 256;
 constexpr int kQueueMaxElog = 3;
 constexpr int kQueueTile = kQueueThreads << kQueueMaxElog;  // 2048 values
-// data tile plus at most as many staged twiddles: within the default 64 KiB
-constexpr size_t kQueueMaxLds = 2 * sizeof(U128) * kQueueTile;
+// padded data tile plus at most as many padded staged twiddles
+constexpr size_t kQueueMaxLds = 80 * 1024;
 
 enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
 
@@ -50,7 +50,23 @@ struct DevDesc {
     U128 *y;
     U128 *tmp;
     const U128 *tw;
+    const R31 *twr;          // digit copy of tw, refreshed by each execution
+    const uint32_t *twflag;  // nonzero when some entry of tw (past entry 0) is not small
 };
+
+// Conversion of the distinct twiddle tables of a plan to digits.
+struct TwJob {
+    const U128 *src;
+    R31 *dst;
+    uint32_t *flag;
+    uint64_t n;
+};
+struct TwChunk {
+    uint32_t job;
+    uint32_t count;
+    uint64_t off;
+};
+constexpr int kTwChunk = 4096;
 
 struct DevMul {
     const U128 *a;
@@ -75,15 +91,36 @@ struct QueueCtl {
     uint32_t pad[2];
 };
 
-// One transform item: load, stage twiddles, network, store.
+__global__ __launch_bounds__(256) void tw_flags_clear_kernel(uint32_t *flags, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) flags[i] = 0;
+}
+
+__global__ __launch_bounds__(256) void tw_convert_kernel(const TwChunk *__restrict__ chunks,
+                                                         const TwJob *__restrict__ jobs)
+{
+    const TwChunk ch = chunks[blockIdx.x];
+    const TwJob jb = jobs[ch.job];
+    bool big = false;
+    for (uint32_t e = threadIdx.x; e < ch.count; e += 256) {
+        const uint64_t i = ch.off + e;
+        const U128 a = ld16(jb.src + i);
+        big |= (i != 0) && is_big(a);  // entry 0 is never read by the network
+        jb.dst[i] = to_r31(a);
+    }
+    if (__syncthreads_or(big) && threadIdx.x == 0) atomicOr(jb.flag, 1u);
+}
+
+// One transform item: load, network, store.
 template <int ELOG>
 __device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, bool slow,
                                                  const Mod &md)
 {
     if (!slow)
-        run_group_network<ELOG, true, true>(lds, tv, k, S, mbase, md);
+        run_group_network<ELOG, TW_GLOBAL, true>(lds, tv, k, S, mbase, md);
     else
-        run_group_network<ELOG, false, true>(lds, tv, k, S, mbase, md);
+        run_group_network<ELOG, TW_SLOW, true>(lds, tv, k, S, mbase, md);
 }
 
 __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkItem &it,
@@ -121,7 +158,7 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
             if (g < nvalid) {
                 const U128 a = ld16(s_x[g] + ti);
                 big |= is_big(a);
-                lds[e] = a;
+                lds[lidx(e, k)] = a;
             }
         } else {
             const int g = e & (G - 1), ti = e >> Glog;
@@ -129,21 +166,18 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
                 const size_t M = it.first + (size_t)g;
                 const U128 a = ld16(src + ((size_t)ti << mlog) + M);
                 big |= is_big(a);
-                lds[(g << k) + ti] = a;
+                lds[lidx((g << k) + ti, k)] = a;
             }
         }
     }
     const uint32_t mbase = (uint32_t)it.first;
-    const int wlog = Glog < S ? Glog : S;
-    const uint32_t W = 1u << wlog;
-    const uint32_t P0start = (wlog == Glog) ? (mbase & ((1u << S) - 1u)) : 0u;
-    R31 *tws = reinterpret_cast<R31 *>(lds + ((size_t)G << k));
-    big |= stage_twiddles(tws, d0.tw, S, k, W, P0start);
+    big |= *d0.twflag != 0;
     const bool slow = __syncthreads_or(big) || md.pm == 0;
     TwView tv;
-    tv.lds = tws;
+    tv.lds = nullptr;
+    tv.gr31 = d0.twr;
     tv.glob = d0.tw;
-    tv.W = W;
+    tv.W = 1;
     switch (elog) {
     case 1:
         run_network_elog<1>(lds, tv, k, S, mbase, slow, md);
@@ -160,9 +194,9 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
         const int e = threadIdx.x + j * kQueueThreads;
         if (mlog == 0) {
             const int g = e >> k;
-            if (g < nvalid) st16(s_y[g] + (e & ((1 << k) - 1)), lds[e]);
+            if (g < nvalid) st16(s_y[g] + (e & ((1 << k) - 1)), lds[lidx(e, k)]);
         } else {
-            if ((e >> k) < nvalid) st16(dst + (it.first << k) + e, lds[e]);
+            if ((e >> k) < nvalid) st16(dst + (it.first << k) + e, lds[lidx(e, k)]);
         }
     }
 }
@@ -176,17 +210,12 @@ __device__ void do_mul_item(const WorkItem &it, const DevMul *__restrict__ muls,
         if (e < it.nvalid) {
             const size_t i = it.first + e;
             const U128 a = ld16(mu.a + i), b = ld16(mu.b + i);
-            U128 r;
-            if (md.pm && below_2_124(a, b))
-                r = mulmod_r31(a, to_r31(b), md.c);
-            else
-                r = mulmod_ref(a, b, md.m, md.mu);
-            st16(mu.c + i, r);
+            st16(mu.c + i, md.pm ? mulmod_elem<true>(a, b, md) : mulmod_elem<false>(a, b, md));
         }
     }
 }
 
-__global__ __launch_bounds__(kQueueThreads) void ntt_queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
+__global__ __launch_bounds__(kQueueThreads, 3) void ntt_queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
                                                                    const DevDesc *__restrict__ descs,
                                                                    const DevMul *__restrict__ muls, QueueCtl *ctl,
                                                                    Mod md)
@@ -242,6 +271,11 @@ struct bntt_batch_plan {
     bntt::QueueCtl *ctl_d = nullptr;
     U128 *tmp_d = nullptr;
     size_t tmp_bytes = 0;
+    bntt::R31 *twr_d = nullptr;          // digit copies of the distinct tables
+    uint32_t *twflag_d = nullptr;        // one flag per distinct table
+    bntt::TwJob *twjobs_d = nullptr;
+    bntt::TwChunk *twchunks_d = nullptr;
+    uint32_t ntwjobs = 0, ntwchunks = 0;
     size_t nitems = 0;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -301,13 +335,24 @@ void free_plan(bntt_batch_plan *p)
     (void)hipFree(p->muls_d);
     (void)hipFree(p->ctl_d);
     (void)hipFree(p->tmp_d);
+    (void)hipFree(p->twr_d);
+    (void)hipFree(p->twflag_d);
+    (void)hipFree(p->twjobs_d);
+    (void)hipFree(p->twchunks_d);
     delete p;
 }
 
-size_t item_lds_bytes(int k, int S, int Glog) { return bntt::pass_lds_bytes(k, S, Glog); }
+size_t item_lds_bytes(int k, int Glog) { return sizeof(U128) * bntt::lds_data_slots(k, Glog); }
 
 int launch_levels(bntt_batch_plan *p, hipStream_t s)
 {
+    if (p->ntwjobs) {
+        hipLaunchKernelGGL(bntt::tw_flags_clear_kernel, dim3((p->ntwjobs + 255) / 256), dim3(256), 0, s, p->twflag_d,
+                           p->ntwjobs);
+        hipLaunchKernelGGL(bntt::tw_convert_kernel, dim3(p->ntwchunks), dim3(256), 0, s, p->twchunks_d, p->twjobs_d);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return bntt_set_hip_error(e, "twiddle conversion launch");
+    }
     for (size_t l = 0; l < p->level_items.size(); l++) {
         if (p->level_items[l] == 0) continue;
         hipLaunchKernelGGL(bntt::ntt_queue_kernel, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
@@ -379,6 +424,59 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             }
             tmp_off[i] = tmp_elems;
             tmp_elems += n;
+        }
+    }
+    // distinct twiddle tables -> digit copies and flags
+    {
+        std::unordered_map<const void *, uint32_t> table_id;
+        std::vector<uint64_t> table_n;
+        std::vector<const U128 *> table_src;
+        std::vector<uint32_t> desc_table(ntasks);
+        for (size_t i = 0; i < ntasks; i++) {
+            auto it = table_id.find(tasks[i].tw);
+            const uint64_t n = (uint64_t)1 << tasks[i].log2n;
+            if (it == table_id.end()) {
+                const uint32_t id = (uint32_t)table_n.size();
+                table_id.emplace(tasks[i].tw, id);
+                table_n.push_back(n);
+                table_src.push_back(static_cast<const U128 *>(tasks[i].tw));
+                desc_table[i] = id;
+            } else {
+                desc_table[i] = it->second;
+                if (table_n[it->second] < n) table_n[it->second] = n;  // one table serving several sizes
+            }
+        }
+        const uint32_t nt = (uint32_t)table_n.size();
+        std::vector<uint64_t> toff(nt);
+        uint64_t tot = 0;
+        for (uint32_t t = 0; t < nt; t++) {
+            toff[t] = tot;
+            tot += table_n[t];
+        }
+        if (nt) {
+            if (hipMalloc(&p->twr_d, tot * sizeof(bntt::R31)) != hipSuccess ||
+                hipMalloc(&p->twflag_d, nt * sizeof(uint32_t)) != hipSuccess) {
+                free_plan(p);
+                return bntt_set_error(BNTT_ENOMEM, "hipMalloc for twiddle digit tables failed");
+            }
+            std::vector<bntt::TwJob> jobs(nt);
+            std::vector<bntt::TwChunk> chunks;
+            for (uint32_t t = 0; t < nt; t++) {
+                jobs[t] = bntt::TwJob{table_src[t], p->twr_d + toff[t], p->twflag_d + t, table_n[t]};
+                for (uint64_t o = 0; o < table_n[t]; o += bntt::kTwChunk)
+                    chunks.push_back(
+                        bntt::TwChunk{t, (uint32_t)std::min<uint64_t>(bntt::kTwChunk, table_n[t] - o), o});
+            }
+            if ((rc = upload(&p->twjobs_d, jobs)) || (rc = upload(&p->twchunks_d, chunks))) {
+                free_plan(p);
+                return rc;
+            }
+            p->ntwjobs = nt;
+            p->ntwchunks = (uint32_t)chunks.size();
+            for (size_t i = 0; i < ntasks; i++) {
+                descs[i].twr = p->twr_d + toff[desc_table[i]];
+                descs[i].twflag = p->twflag_d + desc_table[i];
+            }
         }
     }
     if (tmp_elems) {
@@ -523,7 +621,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         }
         size_t lds = 0;
         for (const WorkItem &w : lv)
-            if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, item_lds_bytes(w.k, w.S, w.Glog));
+            if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, item_lds_bytes(w.k, w.Glog));
         int per_cu = 0;
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::ntt_queue_kernel, bntt::kQueueThreads, lds);
         if (per_cu < 1) per_cu = 1;
@@ -547,6 +645,13 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             free_plan(p);
             return bntt_set_error(BNTT_EINVAL, "internal: batch item exceeds the LDS budget");
         }
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::ntt_queue_kernel),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)bntt::kQueueMaxLds);
+    if (attr != hipSuccess) {
+        free_plan(p);
+        return bntt_set_hip_error(attr, "hipFuncSetAttribute");
+    }
     if ((rc = upload(&p->items_d, all)) || (rc = upload(&p->descs_d, packed)) || (rc = upload(&p->muls_d, mv))) {
         free_plan(p);
         return rc;

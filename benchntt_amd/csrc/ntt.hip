@@ -44,7 +44,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
             const size_t M = gamma & (((size_t)1 << mlog) - 1);
             const U128 a = ld16(x + (tr << L) + ((size_t)ti << mlog) + M);
             big |= is_big(a);
-            lds[(g << k) + ti] = a;
+            lds[lidx((g << k) + ti, k)] = a;
         }
     }
 
@@ -53,18 +53,19 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     const int wlog = Glog < S ? Glog : S;
     const uint32_t W = 1u << wlog;
     const uint32_t P0start = (wlog == Glog) ? (mbase & ((1u << S) - 1u)) : 0u;
-    R31 *tws = reinterpret_cast<R31 *>(lds + ((size_t)G << k));
+    R31 *tws = reinterpret_cast<R31 *>(lds + lds_data_slots(k, Glog));
     big |= stage_twiddles(tws, tw, S, k, W, P0start);
     const bool slow = __syncthreads_or(big) || !PM;
 
     TwView tv;
     tv.lds = tws;
+    tv.gr31 = nullptr;
     tv.glob = tw;
     tv.W = W;
     if (!slow) {
-        if constexpr (PM) run_group_network<ELOG, true, PM>(lds, tv, k, S, mbase, md);
+        if constexpr (PM) run_group_network<ELOG, TW_LDS, PM>(lds, tv, k, S, mbase, md);
     } else {
-        run_group_network<ELOG, false, PM>(lds, tv, k, S, mbase, md);
+        run_group_network<ELOG, TW_SLOW, PM>(lds, tv, k, S, mbase, md);
     }
 
     // LDS -> HBM: the G groups' outputs are one contiguous range.
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
 #pragma unroll
     for (int j = 0; j < E; j++) {
         const int e = threadIdx.x + j * T;
-        if ((e >> k) < nvalid) st16(y + out0 + e, lds[e]);
+        if ((e >> k) < nvalid) st16(y + out0 + e, lds[lidx(e, k)]);
     }
 }
 

@@ -48,8 +48,22 @@ __device__ __forceinline__ void st16(U128 *p, const U128 &a)
 
 constexpr int kMaxThreads = 512;
 // Dynamic LDS of the largest launch: a 4096-point transform and its 4095
-// staged twiddles (the CU has 160 KiB).
-constexpr size_t kMaxLdsBytes = 128 * 1024;
+// staged twiddles, padded (the CU has 160 KiB).
+constexpr size_t kMaxLdsBytes = 148 * 1024;
+
+// LDS placement.  Data element i of groups of 2^k elements sits at
+//   lidx(i, k) = i + (i >> 3) + (i >> k):
+// one pad slot after every 8 elements spreads the stride-2, -4 and -8
+// write-backs of the register sub-passes over all banks, and one extra slot per
+// group spreads the stride-2^k writes of the multi-pass loads (consecutive
+// lanes take consecutive groups).  Staged twiddle entry e sits at tidx(e).
+__device__ __forceinline__ int lidx(int i, int k) { return i + (i >> 3) + (i >> k); }
+__device__ __forceinline__ int tidx(int e) { return e + (e >> 3); }
+__host__ __device__ inline size_t lds_data_slots(int k, int Glog)
+{
+    const size_t n = (size_t)1 << (Glog + k);
+    return n + n / 8 + ((size_t)1 << Glog);
+}
 
 // A value is "small" when its top limb is below that of q = 2^124 - c for
 // every c < 2^16 (0x0FFFFFFF): then it is below q.  When every input and
@@ -59,52 +73,89 @@ constexpr size_t kMaxLdsBytes = 128 * 1024;
 // literal reference formulas.
 __device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFFFFFu; }
 
-// Twiddle lookup of one register sub-pass.  Stage table r holds, for the W
-// distinct group prefixes of this workgroup, the entries
-// tw[2^(S+r) + ((P0_start + rel) << r) + j], rel < W, j < 2^r, at LDS offset
-// W*(2^r - 1) + (rel << r) + j.
+// Where a register sub-pass finds its twiddles.
+//   TW_LDS:    staged digit tables in LDS.  Stage table r holds, for the W
+//              distinct group prefixes of the workgroup, the entries
+//              tw[2^(S+r) + ((P0_start + rel) << r) + j], rel < W, j < 2^r, at
+//              offset W*(2^r - 1) + (rel << r) + j (placed at tidx()).
+//   TW_GLOBAL: a digit copy of the whole table in device memory, indexed like
+//              the table itself (cached in L1/L2).
+//   TW_SLOW:   the caller's table, literal reference arithmetic.
+enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2 };
+
 struct TwView {
-    const R31 *lds;     // staged digit tables (fast path)
-    const U128 *glob;   // the caller's table (literal path)
-    uint32_t W;         // distinct prefixes staged
+    const R31 *lds;     // TW_LDS tables
+    const R31 *gr31;    // TW_GLOBAL table
+    const U128 *glob;   // the caller's table (TW_SLOW)
+    uint32_t W;         // distinct prefixes staged (TW_LDS)
 };
 
-// KP stages of the constant-geometry network on 2^KP values held in registers.
-// Local stages r0 .. r0+KP-1 of a group network that starts at global stage S;
-// rel: the group's staged-prefix index; P0: the group's prefix (low S bits of
-// its M); Mlow: low r0 bits of the sub-group index.
-template <int KP, bool FAST, bool PM>
-__device__ __forceinline__ void reg_network(U128 (&v)[1 << KP], const TwView &tv, int S, int r0, uint32_t rel,
-                                            uint32_t P0, uint32_t Mlow, const Mod &md)
+// Digit twiddles of one register sub-pass, fetched before its LDS reads so that
+// their latency overlaps them: stage rp uses t[2^rp - 1 + j], j < 2^rp.
+template <int KP, int MODE>
+__device__ __forceinline__ void fetch_twiddles(R31 (&t)[(1 << KP) - 1], const TwView &tv, int S, int r0,
+                                               uint32_t rel, uint32_t P0, uint32_t Mlow)
+{
+    const uint32_t P = (P0 << r0) | Mlow;
+#pragma unroll
+    for (int rp = 0; rp < KP; rp++) {
+        const int r = r0 + rp;
+        if constexpr (MODE == TW_LDS) {
+            const int tb = (int)(tv.W * ((1u << r) - 1u) + (rel << r) + (Mlow << rp));
+#pragma unroll
+            for (int j = 0; j < (1 << rp); j++) t[(1 << rp) - 1 + j] = tv.lds[tidx(tb + j)];
+        } else {
+            const R31 *tb = tv.gr31 + (1u << (S + r)) + (P << rp);
+#pragma unroll
+            for (int j = 0; j < (1 << rp); j++) t[(1 << rp) - 1 + j] = tb[j];
+        }
+    }
+}
+
+// KP stages of the constant-geometry network on 2^KP values in registers with
+// fetched digit twiddles (fast exact reduction).
+template <int KP>
+__device__ __forceinline__ void reg_network_fast(U128 (&v)[1 << KP], const R31 (&t)[(1 << KP) - 1], const Mod &md)
 {
     constexpr int H = 1 << (KP - 1);
 #pragma unroll
     for (int rp = 0; rp < KP; rp++) {
-        const int r = r0 + rp;
         U128 w[1 << KP];
-        if constexpr (FAST) {
-            const R31 *tb = tv.lds + tv.W * ((1u << r) - 1u) + (rel << r) + (Mlow << rp);
-            R31 t[H];
 #pragma unroll
-            for (int j = 0; j < (1 << rp) && j < H; j++) t[j] = tb[j];
+        for (int i = 0; i < H; i++) {
+            const U128 p = mulmod_r31(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.c);
+            w[2 * i] = addmod(v[i], p, md);
+            w[2 * i + 1] = submod(v[i], p, md);
+        }
 #pragma unroll
-            for (int i = 0; i < H; i++) {
-                const U128 p = mulmod_r31(v[i + H], t[i & ((1 << rp) - 1)], md.c);
-                w[2 * i] = addmod(v[i], p, md);
-                w[2 * i + 1] = submod(v[i], p, md);
-            }
-        } else {
-            const uint32_t P = (P0 << r0) | Mlow;
-            const U128 *tb = tv.glob + (1u << (S + r)) + (P << rp);
-            U128 t[H];
+        for (int i = 0; i < 2 * H; i++) v[i] = w[i];
+    }
+}
+
+// The same stages with the caller's table and the literal reference formulas
+// (used when some operand of the workgroup is not small, or the modulus is not
+// of the fast form).  Local stages r0 .. r0+KP-1 of a group network that starts
+// at global stage S; P0: the group's prefix (low S bits of its M); Mlow: low r0
+// bits of the sub-group index.
+template <int KP, bool PM>
+__device__ __forceinline__ void reg_network_slow(U128 (&v)[1 << KP], const TwView &tv, int S, int r0, uint32_t P0,
+                                                 uint32_t Mlow, const Mod &md)
+{
+    constexpr int H = 1 << (KP - 1);
+    const uint32_t P = (P0 << r0) | Mlow;
 #pragma unroll
-            for (int j = 0; j < (1 << rp) && j < H; j++) t[j] = ld16(tb + j);
+    for (int rp = 0; rp < KP; rp++) {
+        const int r = r0 + rp;
+        const U128 *tb = tv.glob + (1u << (S + r)) + (P << rp);
+        U128 t[H];
 #pragma unroll
-            for (int i = 0; i < H; i++) {
-                const U128 p = mulmod<PM>(t[i & ((1 << rp) - 1)], v[i + H], md);
-                w[2 * i] = addmod(v[i], p, md);
-                w[2 * i + 1] = submod(v[i], p, md);
-            }
+        for (int j = 0; j < (1 << rp) && j < H; j++) t[j] = ld16(tb + j);
+        U128 w[1 << KP];
+#pragma unroll
+        for (int i = 0; i < H; i++) {
+            const U128 p = mulmod<PM>(t[i & ((1 << rp) - 1)], v[i + H], md);
+            w[2 * i] = addmod(v[i], p, md);
+            w[2 * i + 1] = submod(v[i], p, md);
         }
 #pragma unroll
         for (int i = 0; i < 2 * H; i++) v[i] = w[i];
@@ -113,23 +164,38 @@ __device__ __forceinline__ void reg_network(U128 (&v)[1 << KP], const TwView &tv
 
 // One register sub-pass of KP stages over the G groups of 2^k values in LDS.
 // Each thread owns 2^(ELOG-KP) sub-groups.
-template <int ELOG, int KP, bool FAST, bool PM>
+template <int ELOG, int KP, int MODE, bool PM>
 __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, int S, int r0, uint32_t mbase,
                                             const Mod &md)
 {
     constexpr int Q = 1 << (ELOG - KP);
     constexpr int R = 1 << KP;
+    constexpr int NT = MODE == TW_SLOW ? 1 : (1 << KP) - 1;
     const int T = blockDim.x;
     const int sub_log = k - KP;  // sub-groups per group = 2^sub_log
     U128 v[Q][R];
+    R31 t[Q][NT];
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         const int sg = threadIdx.x + q * T;
         const int g = sg >> sub_log;
         const int Mp = sg & ((1 << sub_log) - 1);
-        const U128 *src = lds + (g << k) + Mp;
+        if constexpr (MODE != TW_SLOW) {
+            const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);
+            const uint32_t Mlow = (uint32_t)Mp & ((1u << r0) - 1u);
+            fetch_twiddles<KP, MODE>(t[q], tv, S, r0, (uint32_t)g & (tv.W - 1u), P0, Mlow);
+        }
+        const int src = (g << k) + Mp;
+        if (sub_log >= 3) {
+            // the sub-group's values are 2^sub_log apart inside one group:
+            // their LDS slots are a fixed stride apart
+            const int base = lidx(src, k), stride = (1 << sub_log) + (1 << (sub_log - 3));
 #pragma unroll
-        for (int j = 0; j < R; j++) v[q][j] = src[j << sub_log];
+            for (int j = 0; j < R; j++) v[q][j] = lds[base + j * stride];
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; j++) v[q][j] = lds[lidx(src + (j << sub_log), k)];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -137,18 +203,28 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
         const int sg = threadIdx.x + q * T;
         const int g = sg >> sub_log;
         const int Mp = sg & ((1 << sub_log) - 1);
-        const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);
-        const uint32_t Mlow = (uint32_t)Mp & ((1u << r0) - 1u);
-        const uint32_t rel = (uint32_t)g & (tv.W - 1u);
-        reg_network<KP, FAST, PM>(v[q], tv, S, r0, rel, P0, Mlow, md);
-        U128 *dst = lds + (g << k) + (Mp << KP);
+        if constexpr (MODE != TW_SLOW) {
+            reg_network_fast<KP>(v[q], t[q], md);
+        } else {
+            const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);
+            const uint32_t Mlow = (uint32_t)Mp & ((1u << r0) - 1u);
+            reg_network_slow<KP, PM>(v[q], tv, S, r0, P0, Mlow, md);
+        }
+        const int dst = (g << k) + (Mp << KP);
+        if (KP == 3 && k >= 3) {
+            // 8 consecutive values of one aligned block of 8: consecutive slots
+            const int base = lidx(dst, k);
 #pragma unroll
-        for (int j = 0; j < R; j++) dst[j] = v[q][j];
+            for (int j = 0; j < R; j++) lds[base + j] = v[q][j];
+        } else {
+#pragma unroll
+            for (int j = 0; j < R; j++) lds[lidx(dst + j, k)] = v[q][j];
+        }
     }
     __syncthreads();
 }
 
-template <int ELOG, bool FAST, bool PM>
+template <int ELOG, int MODE, bool PM>
 __device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase,
                                                   const Mod &md)
 {
@@ -156,12 +232,12 @@ __device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, i
     while (r0 < k) {
         const int kp = (k - r0) < ELOG ? (k - r0) : ELOG;
         if (kp == ELOG) {
-            lds_subpass<ELOG, ELOG, FAST, PM>(lds, tv, k, S, r0, mbase, md);
+            lds_subpass<ELOG, ELOG, MODE, PM>(lds, tv, k, S, r0, mbase, md);
         } else if constexpr (ELOG >= 2) {
             if (kp == 1) {
-                lds_subpass<ELOG, 1, FAST, PM>(lds, tv, k, S, r0, mbase, md);
+                lds_subpass<ELOG, 1, MODE, PM>(lds, tv, k, S, r0, mbase, md);
             } else if constexpr (ELOG >= 3) {
-                if (kp == 2) lds_subpass<ELOG, 2, FAST, PM>(lds, tv, k, S, r0, mbase, md);
+                if (kp == 2) lds_subpass<ELOG, 2, MODE, PM>(lds, tv, k, S, r0, mbase, md);
             }
         }
         r0 += kp;
@@ -183,7 +259,7 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
         const uint32_t off = (uint32_t)e - W * ((1u << r) - 1u);
         const U128 a = ld16(tw + (1u << (S + r)) + (P0start << r) + off);
         big |= is_big(a);
-        tws[e] = to_r31(a);
+        tws[tidx(e)] = to_r31(a);
     }
     return big;
 }
@@ -192,7 +268,8 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
 __host__ __device__ inline size_t pass_lds_bytes(int k, int S, int Glog)
 {
     const int wlog = Glog < S ? Glog : S;
-    return (sizeof(U128) << (Glog + k)) + (sizeof(R31) << wlog) * (((size_t)1 << k) - 1);
+    const size_t ntw = ((size_t)1 << wlog) * (((size_t)1 << k) - 1);
+    return sizeof(U128) * lds_data_slots(k, Glog) + sizeof(R31) * (ntw + ntw / 8 + 1);
 }
 
 }  // namespace bntt

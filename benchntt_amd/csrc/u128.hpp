@@ -390,6 +390,20 @@ BNTT_HD U128 mulmod_ref(const U128 &a_, const U128 &b_, const U128 &m_, const U1
 
 BNTT_HD bool below_2_124(const U128 &a, const U128 &b) { return ((a.v[3] | b.v[3]) >> 28) == 0; }
 
+// Elementwise product: the radix-2^31 reduction when the modulus is of the
+// fast form and both operands are below 2^124, the literal formula otherwise.
+// The test is made per wave: a per-lane choice gets if-converted into code
+// that computes both products for every lane (about 370 instead of 75 VALU
+// ops), and keeping the literal path out of line as a call costs every wave a
+// stack allocation, which throttles how many waves are in flight.
+template <bool PM>
+__device__ __forceinline__ U128 mulmod_elem(const U128 &a, const U128 &b, const Mod &md)
+{
+    const bool fast = PM && below_2_124(a, b);
+    if (__all(fast)) return mulmod_r31(b, to_r31(a), md.c);
+    return fast ? mulmod_r31(b, to_r31(a), md.c) : mulmod_ref(a, b, md.m, md.mu);
+}
+
 template <bool PM>
 BNTT_HD U128 mulmod(const U128 &a, const U128 &b, const Mod &md)
 {
