@@ -99,7 +99,7 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
             "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
 
 
-def uniform_rows(torch, np, bn, qn, mun, stream) -> dict:
+def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dict:
     """Plain transform (N=1024, one launch) and uniform batches: 8192 x N=1024
     and N in {16..512} with 2^23 coefficients, one shared seeded twiddle table
     per N, as one persistent work-queue execution (graph replay) against one
@@ -132,7 +132,7 @@ def uniform_rows(torch, np, bn, qn, mun, stream) -> dict:
             if L == 10:
                 msd = time_kernel(torch, lambda: bn.ntt(yb, xb, qn, mun, twl), 20, 3, s2)
                 tl = TaskList(tasks)
-                mse = time_kernel(torch, lambda: launch_each(tl, qn, mun), 2, 1, s2)
+                mse = time_kernel(torch, lambda: launch_each(tl, qn, mun), 2, 1, s2) if per_launch else float("nan")
         key = f"ntt_uniform_{cnt}x{n}"
         out[f"{key}_NTTs_per_s"] = round(cnt / (msp * 1e-3), 1)
         out[f"{key}_Gbutterflies_per_s"] = round(cnt * n * L / 2 / (msp * 1e-3) / 1e9, 2)
@@ -146,7 +146,7 @@ def uniform_rows(torch, np, bn, qn, mun, stream) -> dict:
     return out
 
 
-def uneven_batch(torch, np, bn, qn, mun, stream) -> dict:
+def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dict:
     """20,000 forward NTTs, N log-uniform on 2^4..2^16 in pairs (seeded), one
     twiddle table per size, each pair's pointwise product after its two
     transforms: one persistent work-queue kernel per level, replayed from a
@@ -179,7 +179,7 @@ def uneven_batch(torch, np, bn, qn, mun, stream) -> dict:
         ms_only = time_kernel(torch, only.execute, 5, 2, s2)
         tmp = torch.empty((1 << 16, 2), dtype=torch.int64, device=dev)
         tl = TaskList(tasks)
-        ms_each = time_kernel(torch, lambda: launch_each(tl, qn, mun, tmp), 1, 1, s2)
+        ms_each = time_kernel(torch, lambda: launch_each(tl, qn, mun, tmp), 1, 1, s2) if per_launch else float("nan")
     info = full.info()
     full.close()
     only.close()
@@ -202,6 +202,8 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-per-launch", action="store_true",
+                    help="skip the one-launch-per-transform comparisons (keeps profiler output small)")
     ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven"], default="all",
                     help="time one workload only and print no JSON line (for profiling runs): "
                          "axpy 2^24; copy (a 256 MiB device copy, the traffic calibration kernel); "
@@ -230,9 +232,9 @@ def main() -> int:
     if args.only in ("ntt-uniform", "uneven"):
         qn, mun = C.Q_NTT, C.MU_NTT
         if args.only == "uneven":
-            uneven_batch(torch, np, bn, qn, mun, stream)
+            uneven_batch(torch, np, bn, qn, mun, stream, not args.skip_per_launch)
             return 0
-        uniform_rows(torch, np, bn, qn, mun, stream)
+        uniform_rows(torch, np, bn, qn, mun, stream, not args.skip_per_launch)
         return 0
     if args.only == "copy":
         src = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
@@ -266,8 +268,8 @@ def main() -> int:
             extras[f"{name}_GBps"] = round(n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9, 1)
             log(f"{name} 2^24: {ms * 1e3:.1f} us/call, {extras[f'{name}_GBps']} GB/s")
 
-        extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream))
-        extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream))
+        extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch))
+        extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch))
 
     traffic = load_traffic("axpy_2_24")
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
