@@ -62,6 +62,27 @@ class BatchInfo(ctypes.Structure):
                 ("level_grid", ctypes.c_int32 * 8), ("level_lds_bytes", ctypes.c_uint32 * 8)]
 
 
+class PmRequest(ctypes.Structure):
+    """bntt_pm_request."""
+
+    _fields_ = [("a", ctypes.c_void_p), ("b", ctypes.c_void_p), ("c", ctypes.c_void_p),
+                ("log2n", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class PmConfig(ctypes.Structure):
+    """bntt_pm_config."""
+
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
+                ("max_batch", ctypes.c_int32), ("min_log2n", ctypes.c_int32), ("max_log2n", ctypes.c_int32)]
+
+
+class PmStats(ctypes.Structure):
+    """bntt_pm_stats."""
+
+    _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("max_inflight_requests", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64)]
+
+
 BNTT_PLAN_GRAPH = 1
 
 _P = ctypes.c_void_p
@@ -89,6 +110,11 @@ _SIGS = {
     "bntt_batch_plan_info": [_P, ctypes.POINTER(BatchInfo)],
     "bntt_batch_plan_destroy": [_P],
     "bntt_ntt_launch_each_dev": [ctypes.POINTER(NttTask), _SZ, _PU, _PU, _P, _P],
+    "bntt_pm_server_create": [ctypes.POINTER(ctypes.c_void_p), _PU, ctypes.POINTER(PmConfig)],
+    "bntt_pm_server_destroy": [_P],
+    "bntt_pm_multiply": [_P, ctypes.POINTER(PmRequest), _SZ],
+    "bntt_pm_run_trace": [_P, ctypes.POINTER(PmRequest), ctypes.POINTER(ctypes.c_uint64), _SZ,
+                          ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(PmStats)],
 }
 # entry points that do not return a status
 _OTHER = {

@@ -63,7 +63,7 @@ u128 barrett_mu_251(u128 m)
 
 Mod make_mod(const bntt_u128 *m, const bntt_u128 *mu)
 {
-    Mod md;
+    Mod md{};
     md.m = bntt::make_u128(m->lo, m->hi);
     md.mu = bntt::make_u128(mu ? mu->lo : 0, mu ? mu->hi : 0);
     md.notm = bntt::make_u128(~m->lo, ~m->hi);

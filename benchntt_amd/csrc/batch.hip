@@ -35,61 +35,9 @@
 #include "internal.hpp"
 #include "ntt_core.hpp"
 
+#include "queue.hpp"
+
 namespace bntt {
-
-constexpr int kQueueThreads = 256;
-constexpr int kQueueMaxElog = 3;
-constexpr int kQueueTile = kQueueThreads << kQueueMaxElog;  // 2048 values
-// padded data tile plus at most as many padded staged twiddles
-constexpr size_t kQueueMaxLds = 80 * 1024;
-
-enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
-
-struct DevDesc {
-    const U128 *x;
-    U128 *y;
-    U128 *tmp;
-    const U128 *tw;
-    const R31 *twr;          // digit copy of tw, refreshed by each execution
-    const uint32_t *twflag;  // nonzero when some entry of tw (past entry 0) is not small
-};
-
-// Conversion of the distinct twiddle tables of a plan to digits.
-struct TwJob {
-    const U128 *src;
-    R31 *dst;
-    uint32_t *flag;
-    uint64_t n;
-};
-struct TwChunk {
-    uint32_t job;
-    uint32_t count;
-    uint64_t off;
-};
-constexpr int kTwChunk = 4096;
-
-struct DevMul {
-    const U128 *a;
-    const U128 *b;
-    U128 *c;
-    uint64_t n;
-};
-
-struct WorkItem {
-    uint8_t kind, L, S, k;
-    uint8_t Glog, elog, pass, np;
-    uint32_t desc;    // first descriptor (NTT) or product index (MUL)
-    uint32_t nvalid;  // groups (NTT) or elements (MUL)
-    uint64_t first;   // first group of the pass (NTT multi-pass) or element offset (MUL)
-    uint64_t pad;
-};
-static_assert(sizeof(WorkItem) == 32, "work item layout");
-
-struct QueueCtl {
-    uint32_t ticket;
-    uint32_t done;
-    uint32_t pad[2];
-};
 
 __global__ __launch_bounds__(256) void tw_flags_clear_kernel(uint32_t *flags, uint32_t n)
 {
@@ -109,143 +57,7 @@ __global__ __launch_bounds__(256) void tw_convert_kernel(const TwChunk *__restri
         big |= (i != 0) && is_big(a);  // entry 0 is never read by the network
         jb.dst[i] = to_r31(a);
     }
-    if (__syncthreads_or(big) && threadIdx.x == 0) atomicOr(jb.flag, 1u);
-}
-
-// One transform item: load, network, store.
-template <int ELOG>
-__device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, bool slow,
-                                                 const Mod &md)
-{
-    if (!slow)
-        run_group_network<ELOG, TW_GLOBAL, true>(lds, tv, k, S, mbase, md);
-    else
-        run_group_network<ELOG, TW_SLOW, true>(lds, tv, k, S, mbase, md);
-}
-
-__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkItem &it,
-                            const DevDesc *__restrict__ descs, const Mod &md)
-{
-    const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
-    const int G = 1 << Glog, E = 1 << elog;
-    const int mlog = L - k;
-    const int nvalid = (int)it.nvalid;
-    bool big = false;
-    const DevDesc d0 = descs[it.desc];
-    const U128 *src;
-    U128 *dst;
-    if (it.np == 1) {
-        src = d0.x;
-        dst = d0.y;
-    } else {
-        const int p = it.pass, np = it.np;
-        dst = ((np - 1 - p) % 2 == 0) ? d0.y : d0.tmp;
-        src = (p == 0) ? d0.x : (((np - p) % 2 == 0) ? d0.y : d0.tmp);
-    }
-    if (mlog == 0) {
-        // whole transforms, one per group, each with its own buffers
-        if ((int)threadIdx.x < nvalid) {
-            s_x[threadIdx.x] = descs[it.desc + threadIdx.x].x;
-            s_y[threadIdx.x] = descs[it.desc + threadIdx.x].y;
-        }
-        __syncthreads();
-    }
-    // HBM -> LDS
-    for (int j = 0; j < E; j++) {
-        const int e = threadIdx.x + j * kQueueThreads;
-        if (mlog == 0) {
-            const int g = e >> k, ti = e & ((1 << k) - 1);
-            if (g < nvalid) {
-                const U128 a = ld16(s_x[g] + ti);
-                big |= is_big(a);
-                lds[lidx(e, k)] = a;
-            }
-        } else {
-            const int g = e & (G - 1), ti = e >> Glog;
-            if (g < nvalid) {
-                const size_t M = it.first + (size_t)g;
-                const U128 a = ld16(src + ((size_t)ti << mlog) + M);
-                big |= is_big(a);
-                lds[lidx((g << k) + ti, k)] = a;
-            }
-        }
-    }
-    const uint32_t mbase = (uint32_t)it.first;
-    big |= *d0.twflag != 0;
-    const bool slow = __syncthreads_or(big) || md.pm == 0;
-    TwView tv;
-    tv.lds = nullptr;
-    tv.gr31 = d0.twr;
-    tv.glob = d0.tw;
-    tv.W = 1;
-    switch (elog) {
-    case 1:
-        run_network_elog<1>(lds, tv, k, S, mbase, slow, md);
-        break;
-    case 2:
-        run_network_elog<2>(lds, tv, k, S, mbase, slow, md);
-        break;
-    default:
-        run_network_elog<3>(lds, tv, k, S, mbase, slow, md);
-        break;
-    }
-    // LDS -> HBM
-    for (int j = 0; j < E; j++) {
-        const int e = threadIdx.x + j * kQueueThreads;
-        if (mlog == 0) {
-            const int g = e >> k;
-            if (g < nvalid) st16(s_y[g] + (e & ((1 << k) - 1)), lds[lidx(e, k)]);
-        } else {
-            if ((e >> k) < nvalid) st16(dst + (it.first << k) + e, lds[lidx(e, k)]);
-        }
-    }
-}
-
-__device__ void do_mul_item(const WorkItem &it, const DevMul *__restrict__ muls, const Mod &md)
-{
-    const DevMul mu = muls[it.desc];
-#pragma unroll
-    for (int j = 0; j < kQueueTile / kQueueThreads; j++) {
-        const uint32_t e = threadIdx.x + j * kQueueThreads;
-        if (e < it.nvalid) {
-            const size_t i = it.first + e;
-            const U128 a = ld16(mu.a + i), b = ld16(mu.b + i);
-            st16(mu.c + i, md.pm ? mulmod_elem<true>(a, b, md) : mulmod_elem<false>(a, b, md));
-        }
-    }
-}
-
-__global__ __launch_bounds__(kQueueThreads, 3) void ntt_queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
-                                                                   const DevDesc *__restrict__ descs,
-                                                                   const DevMul *__restrict__ muls, QueueCtl *ctl,
-                                                                   Mod md)
-{
-    extern __shared__ U128 lds[];
-    __shared__ uint32_t s_ticket;
-    __shared__ const U128 *s_x[kQueueThreads];  // buffers of the transforms of a packed item
-    __shared__ U128 *s_y[kQueueThreads];
-    for (;;) {
-        if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->ticket, 1u);
-        __syncthreads();
-        const uint32_t t = s_ticket;
-        __syncthreads();
-        if (t >= nitems) break;
-        const WorkItem it = items[t];
-        if (it.kind == ITEM_NTT)
-            do_ntt_item(lds, s_x, s_y, it, descs, md);
-        else
-            do_mul_item(it, muls, md);
-    }
-    // every workgroup has drawn a ticket past the end; the last one to leave
-    // rearms the counters for the next launch of this level or the next level
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t d = atomicAdd(&ctl->done, 1u);
-        if (d == gridDim.x - 1) {
-            atomicExch(&ctl->ticket, 0u);
-            atomicExch(&ctl->done, 0u);
-        }
-    }
+    if (__syncthreads_or(big) && threadIdx.x == 0 && jb.flag) atomicOr(jb.flag, 1u);
 }
 
 }  // namespace bntt
@@ -355,9 +167,9 @@ int launch_levels(bntt_batch_plan *p, hipStream_t s)
     }
     for (size_t l = 0; l < p->level_items.size(); l++) {
         if (p->level_items[l] == 0) continue;
-        hipLaunchKernelGGL(bntt::ntt_queue_kernel, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
+        hipLaunchKernelGGL(bntt::queue_kernel<false>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
                            p->level_lds[l], s, p->items_d + p->level_offset[l], (uint32_t)p->level_items[l],
-                           p->descs_d, p->muls_d, p->ctl_d, p->md);
+                           p->descs_d, p->muls_d, nullptr, p->ctl_d, p->md);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "queue kernel launch");
     }
@@ -623,7 +435,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         for (const WorkItem &w : lv)
             if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, item_lds_bytes(w.k, w.Glog));
         int per_cu = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::ntt_queue_kernel, bntt::kQueueThreads, lds);
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<false>, bntt::kQueueThreads, lds);
         if (per_cu < 1) per_cu = 1;
         const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), (size_t)grid_cap * per_cu);
         p->level_offset.push_back(all.size());
@@ -645,7 +457,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             free_plan(p);
             return bntt_set_error(BNTT_EINVAL, "internal: batch item exceeds the LDS budget");
         }
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::ntt_queue_kernel),
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<false>),
                                                        hipFuncAttributeMaxDynamicSharedMemorySize,
                                                        (int)bntt::kQueueMaxLds);
     if (attr != hipSuccess) {

@@ -81,7 +81,10 @@ __device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFF
 //   TW_GLOBAL: a digit copy of the whole table in device memory, indexed like
 //              the table itself (cached in L1/L2).
 //   TW_SLOW:   the caller's table, literal reference arithmetic.
-enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2 };
+//   TW_MONT:   like TW_GLOBAL, but the digit table holds tw * 2^124 mod q and
+//              the arithmetic is the exact Montgomery product with canonical
+//              add/sub (polynomial-multiply path, any odd q < 2^122).
+enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2, TW_MONT = 3 };
 
 struct TwView {
     const R31 *lds;     // TW_LDS tables
@@ -126,6 +129,25 @@ __device__ __forceinline__ void reg_network_fast(U128 (&v)[1 << KP], const R31 (
             const U128 p = mulmod_r31(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.c);
             w[2 * i] = addmod(v[i], p, md);
             w[2 * i + 1] = submod(v[i], p, md);
+        }
+#pragma unroll
+        for (int i = 0; i < 2 * H; i++) v[i] = w[i];
+    }
+}
+
+// The same stages with Montgomery-form digit twiddles: montmul(v, tw*R) = v*tw.
+template <int KP>
+__device__ __forceinline__ void reg_network_mont(U128 (&v)[1 << KP], const R31 (&t)[(1 << KP) - 1], const Mod &md)
+{
+    constexpr int H = 1 << (KP - 1);
+#pragma unroll
+    for (int rp = 0; rp < KP; rp++) {
+        U128 w[1 << KP];
+#pragma unroll
+        for (int i = 0; i < H; i++) {
+            const U128 p = montmul_r31(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.mq);
+            w[2 * i] = addmod_canon(v[i], p, md.mq.ql);
+            w[2 * i + 1] = submod_canon(v[i], p, md.mq.ql);
         }
 #pragma unroll
         for (int i = 0; i < 2 * H; i++) v[i] = w[i];
@@ -203,7 +225,9 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
         const int sg = threadIdx.x + q * T;
         const int g = sg >> sub_log;
         const int Mp = sg & ((1 << sub_log) - 1);
-        if constexpr (MODE != TW_SLOW) {
+        if constexpr (MODE == TW_MONT) {
+            reg_network_mont<KP>(v[q], t[q], md);
+        } else if constexpr (MODE != TW_SLOW) {
             reg_network_fast<KP>(v[q], t[q], md);
         } else {
             const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);

@@ -1,0 +1,638 @@
+// serve.hip -- negacyclic polynomial multiplication c = a * b mod (x^N + 1, q)
+// with continuous batching: no counterpart in the reference (whose ntt() is a
+// single forward transform); built from the same forward network.
+//
+// Pipeline of one request (N = 2^L, q odd < 2^122 with 2^(L+1) | q - 1), all
+// products exact Montgomery products (R = 2^124) so no value leaves [0, q):
+//   FA = F(a; psi table), FB = F(b; psi table)      F = the forward network
+//   G[k] = FA[brv k] * FB[brv k] * R^-1               (ITEM_GMUL)
+//   Y    = F(G; inverse-root table)
+//   c[brv i] = Y[i] * S[i] * R^-1                     (ITEM_SCAT)
+// with tw_psi[2^s + brv_s(j)] = psi_{2^(s+2)}^(2j+1) (the network is then the
+// negacyclic NTT in bit-reversed order), tw_inv[2^s + brv_s(j)] =
+// omega_{2^(s+1)}^-j (the cyclic inverse DFT in bit-reversed order), and
+// S[i] = N^-1 psi_{2N}^-brv(i) R^2: for a bit-reversed-output network F,
+// a_j = N^-1 psi^-j sum_k F(a)[brv k] omega^-jk, i.e. the inverse is F with
+// inverse cyclic roots applied to the bit-reversed product, then a twist.
+// Twiddle entries are stored as tw*R mod q in digits, so that the network's
+// montmul(v, tw*R) = v*tw stays in the ordinary domain; the R^-1 of the
+// pointwise product is cancelled by the extra R in S.
+//
+// A request needs 4N residues of intermediates (FA, FB, G, Y; multi-pass
+// transforms borrow the buffer that is idle at the time), allocated from an
+// arena of a fixed HBM budget.  The trace runner admits arrived requests in
+// order while the arena has room, launches each batch as six persistent
+// work-queue launches on one of several streams, and retires batches as their
+// events complete, so batches of different arrivals overlap on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <thread>
+#include <vector>
+
+#include "../../include/bntt.h"
+#include "internal.hpp"
+#include "queue.hpp"
+
+using bntt::DevDesc;
+using bntt::DevPoint;
+using bntt::Mod;
+using bntt::R31;
+using bntt::U128;
+using bntt::WorkItem;
+
+namespace {
+
+typedef unsigned __int128 u128;
+
+U128 u128_to_limbs(u128 v) { return bntt::make_u128((uint64_t)v, (uint64_t)(v >> 64)); }
+u128 limbs_to_u128(const U128 &a) { return ((u128)bntt::hi64(a) << 64) | bntt::lo64(a); }
+
+// Host Montgomery arithmetic over the device formulas (compiled for the host).
+struct HostMont {
+    bntt::MontQ mq;
+    U128 r1;   // R mod q  (Montgomery form of 1)
+    R31 r2;    // R^2 mod q in digits
+    u128 q;
+
+    bool init(u128 qv)
+    {
+        q = qv;
+        if (!(q & 1) || (q >> 122) != 0 || q < 3) return false;
+        // q^-1 mod 2^128 by Newton iteration, then -q^-1 mod 2^124
+        u128 x = q;
+        for (int i = 0; i < 7; i++) x *= (u128)2 - q * x;
+        const u128 qinv = ((~x + 1) & (((u128)1 << 124) - 1));
+        mq.q = bntt::to_r31(u128_to_limbs(q));
+        mq.qinv = bntt::to_r31(u128_to_limbs(qinv));
+        mq.ql = u128_to_limbs(q);
+        // R mod q and R^2 mod q by doubling
+        u128 v = 1;
+        for (int i = 0; i < 248; i++) {
+            v <<= 1;
+            if (v >= q) v -= q;
+            if (i == 123) r1 = u128_to_limbs(v);
+        }
+        r2 = bntt::to_r31(u128_to_limbs(v));
+        return true;
+    }
+    U128 to_m(u128 x) const { return bntt::montmul_r31(u128_to_limbs(x % q), r2, mq); }
+    U128 mul(const U128 &a, const U128 &b) const { return bntt::montmul_r31(a, bntt::to_r31(b), mq); }
+    u128 from_m(const U128 &a) const
+    {
+        R31 one{};
+        one.d[0] = 1;
+        return limbs_to_u128(bntt::montmul_r31(a, one, mq));
+    }
+    U128 pow(U128 base, u128 e) const
+    {
+        U128 r = r1;
+        while (e) {
+            if (e & 1) r = mul(r, base);
+            base = mul(base, base);
+            e >>= 1;
+        }
+        return r;
+    }
+};
+
+uint32_t brv(uint32_t i, int bits) { return bits ? (__builtin_bitreverse32(i) >> (32 - bits)) : 0; }
+
+struct SizeTables {
+    R31 *fwd = nullptr;    // tw_psi * R, digits
+    R31 *inv = nullptr;    // tw_inv * R, digits
+    R31 *scale = nullptr;  // N^-1 psi^-brv(i) R^2, digits
+};
+
+// First-fit allocator over [0, size) in 256-byte units.
+class Arena {
+  public:
+    void reset(size_t units)
+    {
+        free_.clear();
+        if (units) free_[0] = units;
+        total_ = units;
+        used_ = 0;
+    }
+    bool alloc(size_t units, size_t *off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= units) {
+                *off = it->first;
+                const size_t rest = it->second - units, start = it->first + units;
+                free_.erase(it);
+                if (rest) free_[start] = rest;
+                used_ += units;
+                return true;
+            }
+        }
+        return false;
+    }
+    void release(size_t off, size_t units)
+    {
+        auto it = free_.emplace(off, units).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+        used_ -= units;
+    }
+    size_t total() const { return total_; }
+
+  private:
+    std::map<size_t, size_t> free_;
+    size_t total_ = 0, used_ = 0;
+};
+
+constexpr size_t kUnit = 256;
+
+int passes_for(int L) { return L <= 11 ? 1 : (L + 7) / 8; }
+int elog_for(int L) { return L < 3 ? L : 3; }
+
+size_t request_units(int L) { return ((((size_t)4) << L) * sizeof(U128) + kUnit - 1) / kUnit; }
+
+// Upper bound of work items one request contributes.
+size_t items_per_request(int L)
+{
+    const int np = passes_for(L);
+    if (np == 1) return 3 + 2 * (((size_t)1 << L) + bntt::kQueueTile - 1) / bntt::kQueueTile;
+    size_t per_transform = 0;
+    for (int p = 0, S = 0; p < np; p++) {
+        const int k = (L - S) / (np - p);
+        const int Glog = 8 + elog_for(L) - k;
+        per_transform += (((size_t)1 << (L - k)) + ((size_t)1 << Glog) - 1) >> Glog;
+        S += k;
+    }
+    return 3 * per_transform + 2 * (((size_t)1 << L) + bntt::kQueueTile - 1) / bntt::kQueueTile;
+}
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bntt::QueueCtl *ctl = nullptr;
+    WorkItem *items_d = nullptr;
+    DevDesc *descs_d = nullptr;
+    DevPoint *pts_d = nullptr;
+    char *stage_h = nullptr;  // pinned staging: items | descs | points
+    size_t cap_items = 0, cap_descs = 0, cap_pts = 0;
+    bool busy = false;
+    std::vector<size_t> reqs;                          // request indices of the batch
+    std::vector<std::pair<size_t, size_t>> allocs;     // arena blocks
+};
+
+}  // namespace
+
+struct bntt_pm_server {
+    int dev = -1;
+    Mod md{};
+    HostMont hm;
+    int min_l = 0, max_l = 0, max_batch = 0, nslots = 0;
+    SizeTables tabs[bntt::kMaxLog2 + 1];
+    char *arena_d = nullptr;
+    Arena arena;
+    std::vector<Slot> slots;  // nslots for traces + 1 for bntt_pm_multiply
+    int grid_cap = 0;
+};
+
+namespace {
+
+void destroy_server(bntt_pm_server *s)
+{
+    if (!s) return;
+    (void)hipDeviceSynchronize();
+    for (auto &t : s->tabs) {
+        (void)hipFree(t.fwd);
+        (void)hipFree(t.inv);
+        (void)hipFree(t.scale);
+    }
+    for (auto &sl : s->slots) {
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.done) (void)hipEventDestroy(sl.done);
+        (void)hipFree(sl.ctl);
+        (void)hipFree(sl.items_d);
+        (void)hipFree(sl.descs_d);
+        (void)hipFree(sl.pts_d);
+        if (sl.stage_h) (void)hipHostFree(sl.stage_h);
+    }
+    (void)hipFree(s->arena_d);
+    delete s;
+}
+
+int build_tables(bntt_pm_server *s, int L)
+{
+    const HostMont &hm = s->hm;
+    const u128 q = hm.q;
+    int v = 0;
+    while (((q - 1) >> v & 1) == 0) v++;
+    if (L + 1 > v) return bntt_set_error(BNTT_EINVAL, "q - 1 has too few factors of two for this size");
+    // generator of the 2-power part: a quadratic non-residue g, w = g^((q-1)/2^v)
+    const U128 minus_one = hm.to_m(q - 1);
+    U128 w{};
+    bool found = false;
+    for (u128 g = 2; g < 1000 && !found; g++) {
+        const U128 gm = hm.to_m(g);
+        const U128 e = hm.pow(gm, (q - 1) >> 1);
+        if (std::memcmp(&e, &minus_one, sizeof e) == 0) {
+            w = hm.pow(gm, (q - 1) >> v);
+            found = true;
+        }
+    }
+    if (!found) return bntt_set_error(BNTT_EINVAL, "no quadratic non-residue below 1000 (is q prime?)");
+    const size_t n = (size_t)1 << L;
+    std::vector<R31> fwd(n), inv(n), scale(n);
+    fwd[0] = bntt::to_r31(hm.r1);  // entry 0 is never read
+    inv[0] = fwd[0];
+    for (int st = 0; st < L; st++) {
+        const size_t base = (size_t)1 << st;
+        // r: order 2^(st+2); entries r^(2j+1)
+        const U128 r = hm.pow(w, (u128)1 << (v - st - 2));
+        const U128 r2 = hm.mul(r, r);
+        U128 p = r;
+        // u: inverse of a root of order 2^(st+1); entries u^j
+        const U128 u = hm.pow(w, ((u128)1 << v) - ((u128)1 << (v - st - 1)));
+        U128 pu = hm.r1;
+        for (size_t j = 0; j < base; j++) {
+            const uint32_t at = (uint32_t)base + brv((uint32_t)j, st);
+            // the stored operand is value * R mod q: the Montgomery form itself
+            fwd[at] = bntt::to_r31(p);
+            inv[at] = bntt::to_r31(pu);
+            p = hm.mul(p, r2);
+            pu = hm.mul(pu, u);
+        }
+    }
+    // S[i] = N^-1 psi^-brv(i) R^2 with psi of order 2^(L+1)
+    const U128 psi_inv = hm.pow(w, ((u128)1 << v) - ((u128)1 << (v - L - 1)));
+    const U128 ninv = hm.pow(hm.to_m((q + 1) / 2), (u128)L);
+    U128 acc = ninv;  // Montgomery form of N^-1 psi^-j
+    for (size_t j = 0; j < n; j++) {
+        // stored = value * R^2 = montmul(value * R, R^2)
+        scale[brv((uint32_t)j, L)] = bntt::to_r31(bntt::montmul_r31(acc, hm.r2, hm.mq));
+        acc = hm.mul(acc, psi_inv);
+    }
+    SizeTables &t = s->tabs[L];
+    const size_t bytes = n * sizeof(R31);
+    if (hipMalloc(&t.fwd, bytes) != hipSuccess || hipMalloc(&t.inv, bytes) != hipSuccess ||
+        hipMalloc(&t.scale, bytes) != hipSuccess)
+        return bntt_set_error(BNTT_ENOMEM, "hipMalloc for polynomial-multiply tables failed");
+    if (hipMemcpy(t.fwd, fwd.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t.inv, inv.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t.scale, scale.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        return bntt_set_error(BNTT_EHIP, "uploading polynomial-multiply tables failed");
+    return BNTT_OK;
+}
+
+int make_slot(Slot &sl, size_t cap_items, size_t cap_descs, size_t cap_pts)
+{
+    if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+        return bntt_set_error(BNTT_EHIP, "creating a serving stream failed");
+    sl.cap_items = cap_items;
+    sl.cap_descs = cap_descs;
+    sl.cap_pts = cap_pts;
+    if (hipMalloc(&sl.ctl, sizeof(bntt::QueueCtl)) != hipSuccess ||
+        hipMalloc(&sl.items_d, cap_items * sizeof(WorkItem)) != hipSuccess ||
+        hipMalloc(&sl.descs_d, cap_descs * sizeof(DevDesc)) != hipSuccess ||
+        hipMalloc(&sl.pts_d, cap_pts * sizeof(DevPoint)) != hipSuccess ||
+        hipHostMalloc(&sl.stage_h,
+                      cap_items * sizeof(WorkItem) + cap_descs * sizeof(DevDesc) + cap_pts * sizeof(DevPoint),
+                      hipHostMallocDefault) != hipSuccess)
+        return bntt_set_error(BNTT_ENOMEM, "allocating serving buffers failed");
+    if (hipMemset(sl.ctl, 0, sizeof(bntt::QueueCtl)) != hipSuccess)
+        return bntt_set_error(BNTT_EHIP, "initialising queue counters failed");
+    return BNTT_OK;
+}
+
+struct ReqView {
+    const U128 *a, *b;
+    U128 *c;
+    int L;
+    size_t arena_off;  // in units
+};
+
+// Transforms of one size sharing one table: packed several per item when single-pass.
+void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector<DevDesc> &descs,
+                    const std::vector<DevDesc> &group, int L)
+{
+    const int np = passes_for(L), elog = elog_for(L);
+    if (np == 1) {
+        const int Glog = 8 + elog - L;
+        for (size_t i = 0; i < group.size(); i += ((size_t)1 << Glog)) {
+            WorkItem w{};
+            w.kind = bntt::ITEM_NTT;
+            w.L = w.k = (uint8_t)L;
+            w.Glog = (uint8_t)Glog;
+            w.elog = (uint8_t)elog;
+            w.np = 1;
+            w.desc = (uint32_t)descs.size();
+            w.nvalid = (uint32_t)std::min(group.size() - i, (size_t)1 << Glog);
+            for (size_t j = i; j < i + w.nvalid; j++) descs.push_back(group[j]);
+            lv[lv0].push_back(w);
+        }
+        return;
+    }
+    for (const DevDesc &d : group) {
+        const uint32_t di = (uint32_t)descs.size();
+        descs.push_back(d);
+        for (int p = 0, S = 0; p < np; p++) {
+            const int k = (L - S) / (np - p);
+            const int Glog = 8 + elog - k;
+            const size_t groups = (size_t)1 << (L - k);
+            for (size_t g0 = 0; g0 < groups; g0 += ((size_t)1 << Glog)) {
+                WorkItem w{};
+                w.kind = bntt::ITEM_NTT;
+                w.L = (uint8_t)L;
+                w.S = (uint8_t)S;
+                w.k = (uint8_t)k;
+                w.Glog = (uint8_t)Glog;
+                w.elog = (uint8_t)elog;
+                w.pass = (uint8_t)p;
+                w.np = (uint8_t)np;
+                w.desc = di;
+                w.nvalid = (uint32_t)std::min(groups - g0, (size_t)1 << Glog);
+                w.first = g0;
+                lv[lv0 + p].push_back(w);
+            }
+            S += k;
+        }
+    }
+}
+
+void add_points(std::vector<WorkItem> &lv, std::vector<DevPoint> &pts, const DevPoint &pt, uint8_t kind)
+{
+    const uint32_t pi = (uint32_t)pts.size();
+    pts.push_back(pt);
+    const size_t n = (size_t)1 << pt.log2n;
+    for (size_t off = 0; off < n; off += bntt::kQueueTile) {
+        WorkItem w{};
+        w.kind = kind;
+        w.desc = pi;
+        w.nvalid = (uint32_t)std::min<size_t>(n - off, bntt::kQueueTile);
+        w.first = off;
+        lv.push_back(w);
+    }
+}
+
+// Build the six levels of a batch into the slot's staging buffer and launch them.
+int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
+{
+    // levels: 0,1 forward passes; 2 gather-multiply; 3,4 inverse passes; 5 scale-scatter
+    std::vector<std::vector<WorkItem>> lv(6);
+    std::vector<DevDesc> descs;
+    std::vector<DevPoint> pts;
+    const U128 *arena = reinterpret_cast<const U128 *>(s->arena_d);
+    for (int L = s->min_l; L <= s->max_l; L++) {
+        std::vector<DevDesc> fwd, inv;
+        for (const ReqView &r : reqs) {
+            if (r.L != L) continue;
+            const size_t n = (size_t)1 << L;
+            U128 *base = const_cast<U128 *>(arena) + r.arena_off * (kUnit / sizeof(U128));
+            U128 *FA = base, *FB = base + n, *G = base + 2 * n, *Y = base + 3 * n;
+            fwd.push_back(DevDesc{r.a, FA, Y, nullptr, s->tabs[L].fwd, nullptr});
+            fwd.push_back(DevDesc{r.b, FB, G, nullptr, s->tabs[L].fwd, nullptr});
+            inv.push_back(DevDesc{G, Y, FA, nullptr, s->tabs[L].inv, nullptr});
+            add_points(lv[2], pts, DevPoint{FA, FB, nullptr, G, (uint32_t)L, 0}, bntt::ITEM_GMUL);
+            add_points(lv[5], pts, DevPoint{Y, nullptr, s->tabs[L].scale, r.c, (uint32_t)L, 0}, bntt::ITEM_SCAT);
+        }
+        if (!fwd.empty()) {
+            add_transforms(lv, 0, descs, fwd, L);
+            add_transforms(lv, 3, descs, inv, L);
+        }
+    }
+    size_t nitems = 0;
+    for (auto &v : lv) nitems += v.size();
+    if (nitems > sl.cap_items || descs.size() > sl.cap_descs || pts.size() > sl.cap_pts)
+        return bntt_set_error(BNTT_EINVAL, "internal: batch exceeds the slot capacity");
+    WorkItem *it_h = reinterpret_cast<WorkItem *>(sl.stage_h);
+    DevDesc *de_h = reinterpret_cast<DevDesc *>(sl.stage_h + sl.cap_items * sizeof(WorkItem));
+    DevPoint *pt_h =
+        reinterpret_cast<DevPoint *>(sl.stage_h + sl.cap_items * sizeof(WorkItem) + sl.cap_descs * sizeof(DevDesc));
+    std::vector<size_t> off(6);
+    size_t pos = 0;
+    for (int l = 0; l < 6; l++) {
+        off[l] = pos;
+        std::memcpy(it_h + pos, lv[l].data(), lv[l].size() * sizeof(WorkItem));
+        pos += lv[l].size();
+    }
+    std::memcpy(de_h, descs.data(), descs.size() * sizeof(DevDesc));
+    std::memcpy(pt_h, pts.data(), pts.size() * sizeof(DevPoint));
+    hipError_t e = hipMemcpyAsync(sl.items_d, it_h, nitems * sizeof(WorkItem), hipMemcpyHostToDevice, sl.stream);
+    if (e == hipSuccess && !descs.empty())
+        e = hipMemcpyAsync(sl.descs_d, de_h, descs.size() * sizeof(DevDesc), hipMemcpyHostToDevice, sl.stream);
+    if (e == hipSuccess && !pts.empty())
+        e = hipMemcpyAsync(sl.pts_d, pt_h, pts.size() * sizeof(DevPoint), hipMemcpyHostToDevice, sl.stream);
+    if (e != hipSuccess) return bntt_set_hip_error(e, "hipMemcpyAsync of a batch");
+    for (int l = 0; l < 6; l++) {
+        if (lv[l].empty()) continue;
+        size_t lds = 0;
+        for (const WorkItem &w : lv[l])
+            if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, sizeof(U128) * bntt::lds_data_slots(w.k, w.Glog));
+        const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)s->grid_cap);
+        hipLaunchKernelGGL(bntt::queue_kernel<true>, dim3(grid), dim3(bntt::kQueueThreads), lds, sl.stream,
+                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.pts_d, sl.ctl, s->md);
+        e = hipGetLastError();
+        if (e != hipSuccess) return bntt_set_hip_error(e, "serving kernel launch");
+    }
+    e = hipEventRecord(sl.done, sl.stream);
+    if (e != hipSuccess) return bntt_set_hip_error(e, "hipEventRecord");
+    return BNTT_OK;
+}
+
+int check_request(const bntt_pm_server *s, const bntt_pm_request &r)
+{
+    if (r.log2n < s->min_l || r.log2n > s->max_l) return bntt_set_error(BNTT_EINVAL, "request size not served");
+    if (!r.a || !r.b || !r.c ||
+        ((reinterpret_cast<uintptr_t>(r.a) | reinterpret_cast<uintptr_t>(r.b) | reinterpret_cast<uintptr_t>(r.c)) &
+         15u))
+        return bntt_set_error(BNTT_EINVAL, "request pointers must be non-null and 16-byte aligned");
+    if (request_units(r.log2n) > s->arena.total())
+        return bntt_set_error(BNTT_EINVAL, "request needs more than the whole HBM budget");
+    return BNTT_OK;
+}
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_pm_config *cfg)
+{
+    int rc = bntt_ensure_init();
+    if (rc) return rc;
+    if (!out || !q || !cfg) return bntt_set_error(BNTT_EINVAL, "null argument");
+    *out = nullptr;
+    if (cfg->min_log2n < 1 || cfg->max_log2n > 16 || cfg->min_log2n > cfg->max_log2n || cfg->max_batch < 1 ||
+        cfg->max_inflight < 1 || cfg->max_inflight > 8)
+        return bntt_set_error(BNTT_EINVAL, "bad server configuration");
+    bntt_pm_server *s = new bntt_pm_server();
+    hipGetDevice(&s->dev);
+    if (!s->hm.init(((u128)q->hi << 64) | q->lo)) {
+        delete s;
+        return bntt_set_error(BNTT_EINVAL, "q must be odd, at least 3 and below 2^122");
+    }
+    s->md.m = s->hm.mq.ql;
+    s->md.mq = s->hm.mq;
+    s->min_l = cfg->min_log2n;
+    s->max_l = cfg->max_log2n;
+    s->max_batch = cfg->max_batch;
+    s->nslots = cfg->max_inflight;
+    for (int L = s->min_l; L <= s->max_l; L++)
+        if ((rc = build_tables(s, L))) {
+            destroy_server(s);
+            return rc;
+        }
+    const size_t units = cfg->hbm_budget_bytes / kUnit;
+    if (hipMalloc(&s->arena_d, units * kUnit) != hipSuccess) {
+        destroy_server(s);
+        return bntt_set_error(BNTT_ENOMEM, "hipMalloc of the serving arena failed");
+    }
+    s->arena.reset(units);
+    const size_t cap_items = (size_t)s->max_batch * items_per_request(s->max_l) + 64;
+    s->slots.resize(s->nslots + 1);
+    for (auto &sl : s->slots)
+        if ((rc = make_slot(sl, cap_items, 3 * (size_t)s->max_batch, 2 * (size_t)s->max_batch))) {
+            destroy_server(s);
+            return rc;
+        }
+    hipDeviceProp_t prop;
+    hipGetDeviceProperties(&prop, s->dev);
+    int per_cu = 0;
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<true>, bntt::kQueueThreads,
+                                                 sizeof(U128) * bntt::lds_data_slots(8, 3));
+    s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);
+    if (hipDeviceSynchronize() != hipSuccess) {
+        destroy_server(s);
+        return bntt_set_error(BNTT_EHIP, "server set-up failed");
+    }
+    *out = s;
+    return BNTT_OK;
+}
+
+int bntt_pm_server_destroy(bntt_pm_server *s)
+{
+    destroy_server(s);
+    return BNTT_OK;
+}
+
+int bntt_pm_multiply(bntt_pm_server *s, const bntt_pm_request *reqs, size_t n)
+{
+    if (!s || (n && !reqs)) return bntt_set_error(BNTT_EINVAL, "null argument");
+    if (n > (size_t)s->max_batch) return bntt_set_error(BNTT_EINVAL, "more requests than max_batch");
+    Slot &sl = s->slots.back();
+    std::vector<ReqView> views;
+    std::vector<std::pair<size_t, size_t>> allocs;
+    int rc = BNTT_OK;
+    for (size_t i = 0; i < n && !rc; i++) {
+        rc = check_request(s, reqs[i]);
+        if (rc) break;
+        size_t off;
+        const size_t u = request_units(reqs[i].log2n);
+        if (!s->arena.alloc(u, &off)) {
+            rc = bntt_set_error(BNTT_ENOMEM, "requests exceed the HBM budget");
+            break;
+        }
+        allocs.emplace_back(off, u);
+        views.push_back(ReqView{static_cast<const U128 *>(reqs[i].a), static_cast<const U128 *>(reqs[i].b),
+                                static_cast<U128 *>(reqs[i].c), reqs[i].log2n, off});
+    }
+    if (!rc && !views.empty()) {
+        rc = launch_batch(s, sl, views);
+        if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = bntt_set_error(BNTT_EHIP, "batch failed");
+    }
+    for (auto &a : allocs) s->arena.release(a.first, a.second);
+    return rc;
+}
+
+int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint64_t *arrival_ns, size_t n,
+                      uint64_t *done_ns, bntt_pm_stats *stats)
+{
+    if (!s || (n && (!reqs || !arrival_ns || !done_ns))) return bntt_set_error(BNTT_EINVAL, "null argument");
+    for (size_t i = 0; i < n; i++) {
+        const int rc = check_request(s, reqs[i]);
+        if (rc) return rc;
+        if (i && arrival_ns[i] < arrival_ns[i - 1]) return bntt_set_error(BNTT_EINVAL, "arrivals must be sorted");
+    }
+    bntt_pm_stats st{};
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto now_ns = [&]() { return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - t0).count(); };
+    std::deque<size_t> pending;
+    size_t next = 0, completed = 0, inflight_reqs = 0;
+    std::vector<ReqView> views;
+    while (completed < n) {
+        bool progress = false;
+        const uint64_t t = now_ns();
+        while (next < n && arrival_ns[next] <= t) pending.push_back(next++);
+        // retire finished batches
+        for (int k = 0; k < s->nslots; k++) {
+            Slot &sl = s->slots[k];
+            if (!sl.busy) continue;
+            const hipError_t q = hipEventQuery(sl.done);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) return bntt_set_hip_error(q, "serving batch failed");
+            const uint64_t tdone = now_ns();
+            for (size_t r : sl.reqs) done_ns[r] = tdone;
+            for (auto &a : sl.allocs) s->arena.release(a.first, a.second);
+            completed += sl.reqs.size();
+            inflight_reqs -= sl.reqs.size();
+            sl.reqs.clear();
+            sl.allocs.clear();
+            sl.busy = false;
+            progress = true;
+        }
+        // admit arrived requests into a free slot while the budget allows
+        for (int k = 0; k < s->nslots && !pending.empty(); k++) {
+            Slot &sl = s->slots[k];
+            if (sl.busy) continue;
+            views.clear();
+            while (!pending.empty() && (int)views.size() < s->max_batch) {
+                const size_t r = pending.front();
+                const size_t u = request_units(reqs[r].log2n);
+                size_t off;
+                if (!s->arena.alloc(u, &off)) {
+                    st.admission_waits++;
+                    break;
+                }
+                pending.pop_front();
+                sl.allocs.emplace_back(off, u);
+                sl.reqs.push_back(r);
+                views.push_back(ReqView{static_cast<const U128 *>(reqs[r].a), static_cast<const U128 *>(reqs[r].b),
+                                        static_cast<U128 *>(reqs[r].c), reqs[r].log2n, off});
+            }
+            if (views.empty()) break;  // budget full: wait for a batch to retire
+            const int rc = launch_batch(s, sl, views);
+            if (rc) return rc;
+            sl.busy = true;
+            st.batches++;
+            inflight_reqs += views.size();
+            st.max_inflight_requests = std::max<uint64_t>(st.max_inflight_requests, inflight_reqs);
+            progress = true;
+        }
+        if (!progress) {
+            // nothing to do until the next arrival or completion: sleep through
+            // long idle gaps, otherwise poll
+            const uint64_t tn = now_ns();
+            if (next < n && pending.empty() && inflight_reqs == 0 && arrival_ns[next] > tn + 200000)
+                std::this_thread::sleep_for(std::chrono::nanoseconds(arrival_ns[next] - tn - 100000));
+            else
+                std::this_thread::yield();
+        }
+    }
+    st.requests = n;
+    if (stats) *stats = st;
+    return BNTT_OK;
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

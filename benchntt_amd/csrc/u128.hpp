@@ -35,6 +35,18 @@ struct alignas(16) U128 {
     uint32_t v[4];
 };
 
+// A residue as four 31-bit digits: value = d0 + d1*2^31 + d2*2^62 + d3*2^93.
+struct R31 {
+    uint32_t d[4];
+};
+
+// Montgomery constants of an odd modulus q < 2^122 (R = 2^124).
+struct MontQ {
+    R31 q;       // q in digits
+    R31 qinv;    // -q^-1 mod 2^124 in digits
+    U128 ql;     // q in 32-bit limbs
+};
+
 // Modulus descriptor passed to kernels by value.
 struct Mod {
     U128 m;       // modulus
@@ -42,6 +54,7 @@ struct Mod {
     U128 notm;    // ~m (used by the literal addmod)
     uint32_t c;   // 2^124 - m when pm != 0
     uint32_t pm;  // m = 2^124 - c, c < 2^16, mu = floor(2^251/m)
+    MontQ mq;     // exact Montgomery constants (polynomial path only)
 };
 
 BNTT_HD U128 make_u128(uint64_t lo, uint64_t hi)
@@ -215,10 +228,6 @@ BNTT_HD U128 mulmod_pm(const U128 &a, const U128 &b, uint32_t c)
 // stays below 2^64, so each column is a plain chain of multiply-adds with no
 // carry fixups: 16 v_mad_u64_u32 plus one mask and one 64-bit shift per column.
 
-struct R31 {
-    uint32_t d[4];  // value = d0 + d1*2^31 + d2*2^62 + d3*2^93, each d < 2^31
-};
-
 constexpr uint32_t kM31 = 0x7FFFFFFFu;
 
 // 32-bit limbs -> 31-bit digits, for a < 2^124.
@@ -297,6 +306,122 @@ BNTT_HD U128 mulmod_r31(const U128 &a_, const R31 &w, uint32_t c)
     r.v[3] = ge ? (z3 & 0x0FFFFFFFu) : w3;
     return r;
 }
+
+// -------------------------------------- exact Montgomery product, any odd q < 2^122
+//
+// For the polynomial-multiply path (no counterpart in the reference), whose
+// modulus q_116 = 2^115 + 47*2^17 + 1 is not of the 2^124 - c form and for which
+// the reference's MBITS=124 Barrett schedule is not exact.  R = 2^124 (four
+// digits), qinv = -q^-1 mod R.  For a, w < q the result a*w*R^-1 mod q is
+// canonical: u = a*w + m*q with m = (a*w mod R)*qinv mod R is divisible by R
+// and u/R < (q^2 + R*q)/R < 2q, so one conditional subtraction finishes.
+
+BNTT_HD U128 montmul_r31(const U128 &a_, const R31 &w, const MontQ &mq)
+{
+    const R31 a = to_r31(a_);
+    uint64_t t;
+    uint32_t d[8];
+    t = (uint64_t)a.d[0] * w.d[0];
+    d[0] = (uint32_t)t & kM31;
+    t = mad32(a.d[0], w.d[1], t >> 31);
+    t = mad32(a.d[1], w.d[0], t);
+    d[1] = (uint32_t)t & kM31;
+    t = mad32(a.d[0], w.d[2], t >> 31);
+    t = mad32(a.d[1], w.d[1], t);
+    t = mad32(a.d[2], w.d[0], t);
+    d[2] = (uint32_t)t & kM31;
+    t = mad32(a.d[0], w.d[3], t >> 31);
+    t = mad32(a.d[1], w.d[2], t);
+    t = mad32(a.d[2], w.d[1], t);
+    t = mad32(a.d[3], w.d[0], t);
+    d[3] = (uint32_t)t & kM31;
+    t = mad32(a.d[1], w.d[3], t >> 31);
+    t = mad32(a.d[2], w.d[2], t);
+    t = mad32(a.d[3], w.d[1], t);
+    d[4] = (uint32_t)t & kM31;
+    t = mad32(a.d[2], w.d[3], t >> 31);
+    t = mad32(a.d[3], w.d[2], t);
+    d[5] = (uint32_t)t & kM31;
+    t = mad32(a.d[3], w.d[3], t >> 31);
+    d[6] = (uint32_t)t & kM31;
+    d[7] = (uint32_t)(t >> 31);
+    // m = (t mod R) * qinv mod R
+    uint32_t m[4];
+    const R31 &qi = mq.qinv;
+    t = (uint64_t)d[0] * qi.d[0];
+    m[0] = (uint32_t)t & kM31;
+    t = mad32(d[0], qi.d[1], t >> 31);
+    t = mad32(d[1], qi.d[0], t);
+    m[1] = (uint32_t)t & kM31;
+    t = mad32(d[0], qi.d[2], t >> 31);
+    t = mad32(d[1], qi.d[1], t);
+    t = mad32(d[2], qi.d[0], t);
+    m[2] = (uint32_t)t & kM31;
+    t = mad32(d[0], qi.d[3], t >> 31);
+    t = mad32(d[1], qi.d[2], t);
+    t = mad32(d[2], qi.d[1], t);
+    t = mad32(d[3], qi.d[0], t);
+    m[3] = (uint32_t)t & kM31;
+    // u = t + m*q: the low four digits vanish, keep their carries
+    const R31 &q = mq.q;
+    t = mad32(m[0], q.d[0], d[0]);
+    t = mad32(m[0], q.d[1], (t >> 31) + d[1]);
+    t = mad32(m[1], q.d[0], t);
+    t = mad32(m[0], q.d[2], (t >> 31) + d[2]);
+    t = mad32(m[1], q.d[1], t);
+    t = mad32(m[2], q.d[0], t);
+    t = mad32(m[0], q.d[3], (t >> 31) + d[3]);
+    t = mad32(m[1], q.d[2], t);
+    t = mad32(m[2], q.d[1], t);
+    t = mad32(m[3], q.d[0], t);
+    uint32_t r[4];
+    t = mad32(m[1], q.d[3], (t >> 31) + d[4]);
+    t = mad32(m[2], q.d[2], t);
+    t = mad32(m[3], q.d[1], t);
+    r[0] = (uint32_t)t & kM31;
+    t = mad32(m[2], q.d[3], (t >> 31) + d[5]);
+    t = mad32(m[3], q.d[2], t);
+    r[1] = (uint32_t)t & kM31;
+    t = mad32(m[3], q.d[3], (t >> 31) + d[6]);
+    r[2] = (uint32_t)t & kM31;
+    r[3] = (uint32_t)(t >> 31) + d[7];  // u/R < 2q < 2^123: fits, no carry out
+    // digits -> 32-bit limbs, then subtract q once if needed
+    const uint32_t w0 = r[0] | (r[1] << 31), w1 = (r[1] >> 1) | (r[2] << 30);
+    const uint32_t w2 = (r[2] >> 2) | (r[3] << 29), w3 = r[3] >> 3;
+    uint32_t k, z0, z1, z2, z3;
+    z0 = subb(w0, mq.ql.v[0], 0, k);
+    z1 = subb(w1, mq.ql.v[1], k, k);
+    z2 = subb(w2, mq.ql.v[2], k, k);
+    z3 = subb(w3, mq.ql.v[3], k, k);
+    U128 res;
+    res.v[0] = k ? w0 : z0;
+    res.v[1] = k ? w1 : z1;
+    res.v[2] = k ? w2 : z2;
+    res.v[3] = k ? w3 : z3;
+    return res;
+}
+
+// Canonical (a + b) mod q and (a - b) mod q for a, b < q.
+BNTT_HD U128 addmod_canon(const U128 &a, const U128 &b, const U128 &q)
+{
+    uint32_t k, s0, s1, s2, s3, z0, z1, z2, z3, bb;
+    s0 = addc(a.v[0], b.v[0], 0, k);
+    s1 = addc(a.v[1], b.v[1], k, k);
+    s2 = addc(a.v[2], b.v[2], k, k);
+    s3 = addc(a.v[3], b.v[3], k, k);
+    z0 = subb(s0, q.v[0], 0, bb);
+    z1 = subb(s1, q.v[1], bb, bb);
+    z2 = subb(s2, q.v[2], bb, bb);
+    z3 = subb(s3, q.v[3], bb, bb);
+    U128 r;
+    r.v[0] = bb ? s0 : z0;
+    r.v[1] = bb ? s1 : z1;
+    r.v[2] = bb ? s2 : z2;
+    r.v[3] = bb ? s3 : z3;
+    return r;
+}
+
+BNTT_HD U128 submod_canon(const U128 &a, const U128 &b, const U128 &q) { return submod_ref(a, b, q); }
 
 // ---------------------------------------------------- literal reference Barrett
 

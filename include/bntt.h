@@ -158,6 +158,56 @@ int bntt_batch_plan_destroy(bntt_batch_plan *plan);
 int bntt_ntt_launch_each_dev(const bntt_ntt_task *tasks, size_t ntasks, const bntt_u128 *m, const bntt_u128 *mu,
                              void *tmp, void *stream);
 
+/* ------------------------------------------- polynomial multiplication server */
+/*
+ * No counterpart in the reference.  Negacyclic products c = a * b mod
+ * (x^N + 1, q) for an odd prime q < 2^122 with 2^(N's log2 + 1) | q - 1 (the
+ * reference's q_116 = 2^115 + 47*2^17 + 1 serves N up to 2^16), built from the
+ * same forward network with exact Montgomery arithmetic.  Operands and results
+ * are canonical residues (< q) in device memory; twiddle tables stay resident
+ * in the server.  One server is driven by one host thread at a time.  The
+ * server runs on streams of its own that do not wait for other streams:
+ * operands must be in device memory before a call, and c may be read by any
+ * stream after it returns.  A result does not depend on the other requests of
+ * its batch.
+ */
+typedef struct bntt_pm_request {
+    const void *a; /* N residues */
+    const void *b; /* N residues */
+    void *c;       /* N residues out */
+    int32_t log2n;
+    int32_t reserved;
+} bntt_pm_request;
+
+typedef struct bntt_pm_config {
+    uint64_t hbm_budget_bytes; /* arena for intermediates: 64*N bytes per request in flight */
+    int32_t max_inflight;      /* batches on the device at once (1..8 streams) */
+    int32_t max_batch;         /* requests per batch */
+    int32_t min_log2n;         /* sizes served: tables are built at creation */
+    int32_t max_log2n;         /* <= 16 */
+} bntt_pm_config;
+
+typedef struct bntt_pm_stats {
+    uint64_t requests;
+    uint64_t batches;
+    uint64_t max_inflight_requests; /* most requests on the device at once */
+    uint64_t admission_waits;       /* times an arrived request waited for budget */
+} bntt_pm_stats;
+
+typedef struct bntt_pm_server bntt_pm_server;
+
+int bntt_pm_server_create(bntt_pm_server **server, const bntt_u128 *q, const bntt_pm_config *config);
+int bntt_pm_server_destroy(bntt_pm_server *server);
+/* Multiply n <= max_batch requests as one batch; returns when c is written. */
+int bntt_pm_multiply(bntt_pm_server *server, const bntt_pm_request *reqs, size_t n);
+/* Serve requests that arrive at arrival_ns[i] (sorted, ns after the call
+ * starts): arrived requests are admitted in order while the HBM budget has
+ * room, batched, and launched on a free stream while earlier batches still
+ * run; done_ns[i] receives the time request i was seen complete.  Returns when
+ * all are done. */
+int bntt_pm_run_trace(bntt_pm_server *server, const bntt_pm_request *reqs, const uint64_t *arrival_ns, size_t n,
+                      uint64_t *done_ns, bntt_pm_stats *stats);
+
 /* -------------------------------------------------------------- host pointers */
 
 int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m);

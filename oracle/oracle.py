@@ -192,6 +192,53 @@ class Oracle:
         return c
 
 
+    def polymul_ntt(self, a, b, q: int):
+        """Negacyclic product through the exact networks: or_ntt_exact of a and
+        b with the table tw[2^s + k] = psi_s^(2 brv_s(k) + 1) (psi_s of order
+        2^(s+2)), pointwise product, or_intt_exact with the inverse table.  For
+        a prime q with 2N | q - 1; agrees with polymul_schoolbook."""
+        a, b = (np.ascontiguousarray(v, dtype=np.uint64).reshape(-1, 2) for v in (a, b))
+        n = a.shape[0]
+        L = n.bit_length() - 1
+        tw, itw = negacyclic_tables(q, L)
+        fa, fb = self.ntt_exact(a, from_ints(tw), q), self.ntt_exact(b, from_ints(tw), q)
+        prod = from_ints([x * y % q for x, y in zip(to_ints(fa), to_ints(fb))])
+        return self.intt_exact(prod, from_ints(itw), q)
+
+
+def two_adic_root(q: int) -> tuple[int, int]:
+    """(w, v): 2^v is the largest power of two dividing q - 1 and w is a root of
+    unity of order exactly 2^v (g^((q-1)/2^v) for the least quadratic
+    non-residue g; q prime)."""
+    v = ((q - 1) & -(q - 1)).bit_length() - 1
+    g = 2
+    while pow(g, (q - 1) // 2, q) != q - 1:
+        g += 1
+    return pow(g, (q - 1) >> v, q), v
+
+
+def _brv(x: int, bits: int) -> int:
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def negacyclic_tables(q: int, L: int) -> tuple[list[int], list[int]]:
+    """Forward table (negacyclic NTT in bit-reversed order through the forward
+    network) and its entry-wise inverse, for N = 2^L."""
+    w, v = two_adic_root(q)
+    if L + 1 > v:
+        raise ValueError("q - 1 has too few factors of two")
+    n = 1 << L
+    tw, itw = [0] * n, [0] * n
+    for s in range(L):
+        r = pow(w, 1 << (v - s - 2), q)
+        ri = pow(r, q - 2, q)
+        for k in range(1 << s):
+            e = 2 * _brv(k, s) + 1
+            tw[(1 << s) + k] = pow(r, e, q)
+            itw[(1 << s) + k] = pow(ri, e, q)
+    return tw, itw
+
+
 _ORACLE: Oracle | None = None
 
 

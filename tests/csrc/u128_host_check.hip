@@ -11,7 +11,8 @@ using bntt::U128;
 extern "C" {
 
 // op: 0 addmod_ref, 1 submod_ref, 2 mulmod_ref, 3 mulmod_pm, 4 mulmod<true>, 5 mulmod<false>,
-//     6 mulmod_r31 (b converted to digits first)
+//     6 mulmod_r31 (b converted to digits first), 7 montmul_r31 (mu = -q^-1 mod 2^124),
+//     8 addmod_canon, 9 submod_canon
 void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U128 *m, const U128 *mu, uint32_t c)
 {
     bntt::Mod md;
@@ -28,7 +29,17 @@ void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U12
         case 3: out[i] = bntt::mulmod_pm(a[i], b[i], c); break;
         case 4: out[i] = bntt::mulmod<true>(a[i], b[i], md); break;
         case 5: out[i] = bntt::mulmod<false>(a[i], b[i], md); break;
-        default: out[i] = bntt::mulmod_r31(a[i], bntt::to_r31(b[i]), c); break;
+        case 6: out[i] = bntt::mulmod_r31(a[i], bntt::to_r31(b[i]), c); break;
+        case 7: {
+            bntt::MontQ mq;
+            mq.q = bntt::to_r31(*m);
+            mq.qinv = bntt::to_r31(*mu);
+            mq.ql = *m;
+            out[i] = bntt::montmul_r31(a[i], bntt::to_r31(b[i]), mq);
+            break;
+        }
+        case 8: out[i] = bntt::addmod_canon(a[i], b[i], *m); break;
+        default: out[i] = bntt::submod_canon(a[i], b[i], *m); break;
         }
     }
 }

@@ -1,0 +1,187 @@
+"""GPU parity of the polynomial-multiply server (negacyclic c = a * b mod
+(x^N + 1, q)): products against the oracle (schoolbook up to N = 2^9, the
+oracle's exact NTT pipeline above that), closed forms, one request alone
+against the same request inside a continuous-batching trace, admission under a
+tight HBM budget, and argument errors."""
+import numpy as np
+import pytest
+
+from benchntt_amd import config as C
+from benchntt_amd.residues import from_ints, to_ints, uniform
+from conftest import from_dev, to_dev
+from test_oracle import POLY_MODULI
+
+pytestmark = pytest.mark.gpu
+
+UNIT = 256
+
+
+def req_bytes(L):
+    """Arena bytes one request of 2^L points holds while in flight."""
+    return -(-(4 << L) * 16 // UNIT) * UNIT
+
+
+def rand(n, q, seed):
+    if q.bit_length() > 64:
+        return uniform(n, q, seed)
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = np.zeros((n, 2), dtype=np.uint64)
+    out[:, 0] = rng.integers(0, q, size=n, dtype=np.uint64)
+    return out
+
+
+def server(q, **kw):
+    from benchntt_amd.polymul import PolymulServer
+
+    print(f"\nserver q={q:#x} {kw}", flush=True)
+    return PolymulServer(q, **kw)
+
+
+def run_batch(torch, srv, pairs):
+    """One bntt_pm_multiply batch; returns host results."""
+    dev = [(to_dev(torch, a), to_dev(torch, b)) for a, b in pairs]
+    outs = [torch.empty_like(da) for da, _ in dev]
+    srv.multiply([(da, db, dc) for (da, db), dc in zip(dev, outs)])
+    return [from_dev(c) for c in outs]
+
+
+def expect(orc, a, b, q):
+    return orc.polymul_schoolbook(a, b, q) if a.shape[0] <= 512 else orc.polymul_ntt(a, b, q)
+
+
+@pytest.mark.parametrize("name", sorted(POLY_MODULI))
+def test_small_sizes_schoolbook(gpu, orc, name):
+    """Every size 2^1 .. 2^9 (at most the modulus' 2-adicity), two random
+    requests each plus edge operands, all in one batch, against schoolbook."""
+    q, lmax = POLY_MODULI[name]
+    top = min(lmax, 9)
+    srv = server(q, min_log2n=1, max_log2n=top, max_batch=64)
+    pairs = []
+    for L in range(1, top + 1):
+        n = 1 << L
+        pairs.append((rand(n, q, 10 * L), rand(n, q, 10 * L + 1)))
+        pairs.append((rand(n, q, 10 * L + 2), rand(n, q, 10 * L + 3)))
+        pairs.append((from_ints([q - 1] * n), from_ints([q - 1] * n)))
+    got = run_batch(gpu, srv, pairs)
+    for i, ((a, b), c) in enumerate(zip(pairs, got)):
+        assert np.array_equal(c, orc.polymul_schoolbook(a, b, q)), f"request {i} (N={a.shape[0]})"
+    srv.close()
+
+
+@pytest.mark.parametrize("name", ["q116", "q122", "q64"])
+def test_closed_forms(gpu, name):
+    """x * x^(N-1) = -1, 1 * b = b, 0 * b = 0, x^i * x^j = +-x^((i+j) mod N)
+    (pins the twist, bit-reversal and sign of the negacyclic wrap)."""
+    q, _ = POLY_MODULI[name]
+    srv = server(q, min_log2n=3, max_log2n=16, max_batch=64)
+    rng = np.random.default_rng(5)
+    pairs, want = [], []
+    for L in (3, 8, 11, 12, 13, 16):
+        n = 1 << L
+        b = rand(n, q, L)
+        one = from_ints([1] + [0] * (n - 1))
+        pairs += [(one, b), (from_ints([0] * n), b)]
+        want += [b, from_ints([0] * n)]
+        for _ in range(2):
+            i, j = (int(v) for v in rng.integers(0, n, size=2))
+            xi, xj = from_ints([0] * n), from_ints([0] * n)
+            xi[i, 0] = xj[j, 0] = 1
+            w = [0] * n
+            w[(i + j) % n] = 1 if i + j < n else q - 1
+            pairs.append((xi, xj))
+            want.append(from_ints(w))
+    got = run_batch(gpu, srv, pairs)
+    for k, (c, w) in enumerate(zip(got, want)):
+        assert np.array_equal(c, w), f"case {k} (N={w.shape[0]})"
+    srv.close()
+
+
+@pytest.mark.parametrize("name,sizes", [("q116", range(10, 17)), ("q122", (12, 16)), ("q64", (11, 16))])
+def test_large_sizes_oracle(gpu, orc, name, sizes):
+    """Sizes 2^10 .. 2^16 (single- and two-pass transforms) against the
+    oracle's exact NTT pipeline, all in one batch."""
+    q, _ = POLY_MODULI[name]
+    srv = server(q, min_log2n=10, max_log2n=16, max_batch=16)
+    pairs = [(rand(1 << L, q, 100 + L), rand(1 << L, q, 200 + L)) for L in sizes]
+    got = run_batch(gpu, srv, pairs)
+    for (a, b), c in zip(pairs, got):
+        assert np.array_equal(c, orc.polymul_ntt(a, b, q)), f"N={a.shape[0]}"
+    srv.close()
+
+
+def _trace(torch, n, q, seed, lo, hi, gap_ns):
+    rng = np.random.default_rng(seed)
+    logs = rng.integers(lo, hi + 1, size=n)
+    hosts = [(rand(1 << int(L), q, seed * 1000 + 2 * i), rand(1 << int(L), q, seed * 1000 + 2 * i + 1))
+             for i, L in enumerate(logs)]
+    dev = [(to_dev(torch, a), to_dev(torch, b)) for a, b in hosts]
+    arrival = np.cumsum(rng.exponential(gap_ns, size=n)).astype(np.uint64) if gap_ns else np.zeros(n, np.uint64)
+    return logs, hosts, dev, arrival
+
+
+def test_alone_equals_mixed(gpu, orc):
+    """Each request of a Poisson trace with mixed sizes 2^4 .. 2^14, served
+    with continuous batching over several streams, gives the same bits as the
+    same request multiplied alone; a sample is checked against the oracle."""
+    q = C.Q_116
+    srv = server(q, hbm_budget=1 << 30, max_inflight=4, max_batch=16, min_log2n=4, max_log2n=14)
+    logs, hosts, dev, arrival = _trace(gpu, 160, q, 7, 4, 14, 20_000)
+    mixed = [gpu.empty_like(a) for a, _ in dev]
+    done, st = srv.run_trace([(a, b, c) for (a, b), c in zip(dev, mixed)], arrival)
+    assert st["requests"] == 160 and st["batches"] >= 2
+    assert np.all(done >= arrival)
+    for i, ((a, b), c) in enumerate(zip(dev, mixed)):
+        alone = gpu.empty_like(a)
+        srv.multiply([(a, b, alone)])
+        assert gpu.equal(alone, c), f"request {i} (N=2^{int(logs[i])}) differs alone vs mixed"
+    for i in range(0, 160, 16):
+        a, b = hosts[i]
+        assert np.array_equal(from_dev(mixed[i]), expect(orc, a, b, q)), f"request {i}"
+    srv.close()
+
+
+def test_tight_budget_admission(gpu):
+    """A budget of 2.5 requests of 2^12 points: arrived requests wait for
+    budget, never more than two are on the device, and results are unchanged."""
+    q = C.Q_116
+    budget = int(2.5 * req_bytes(12))
+    srv = server(q, hbm_budget=budget, max_inflight=4, max_batch=8, min_log2n=12, max_log2n=12)
+    logs, hosts, dev, arrival = _trace(gpu, 24, q, 9, 12, 12, 0)
+    mixed = [gpu.empty_like(a) for a, _ in dev]
+    done, st = srv.run_trace([(a, b, c) for (a, b), c in zip(dev, mixed)], arrival)
+    assert st["admission_waits"] > 0
+    assert 1 <= st["max_inflight_requests"] <= 2
+    assert st["batches"] >= 12
+    for i, ((a, b), c) in enumerate(zip(dev, mixed)):
+        alone = gpu.empty_like(a)
+        srv.multiply([(a, b, alone)])
+        assert gpu.equal(alone, c), f"request {i}"
+    srv.close()
+
+
+def test_errors(gpu):
+    from benchntt_amd import BnttError
+    from benchntt_amd.polymul import PolymulServer
+
+    q = C.Q_116
+    with pytest.raises(BnttError):
+        PolymulServer(q - 1)  # even
+    with pytest.raises(BnttError):
+        PolymulServer(C.Q_NTT)  # above 2^122
+    with pytest.raises(BnttError):
+        PolymulServer(97, min_log2n=1, max_log2n=5)  # 2^6 does not divide 96
+    with pytest.raises(BnttError):
+        PolymulServer(q, max_log2n=17)
+    srv = PolymulServer(q, hbm_budget=req_bytes(10), min_log2n=4, max_log2n=12)
+    a = to_dev(gpu, rand(1 << 11, q, 1))
+    with pytest.raises(BnttError):
+        srv.multiply([(a, a, gpu.empty_like(a))])  # larger than the whole budget
+    small = to_dev(gpu, rand(8, q, 1))
+    with pytest.raises(BnttError):
+        srv.multiply([(small, small, gpu.empty_like(small))])  # size not served
+    with pytest.raises(ValueError):
+        srv.multiply([(a[:100], a[:100], a[:100].clone())])  # not a power of two
+    ok = to_dev(gpu, rand(1 << 10, q, 2))
+    out = gpu.empty_like(ok)
+    srv.multiply([(ok, ok, out)])  # the server still works after the errors
+    srv.close()

@@ -234,3 +234,50 @@ def test_oracle_matches_reference_primitives_on_edges(orc):
         assert orc.addmod(a, b, q) == ref.scalar_primitive("addmod128", a, b, q)
         assert orc.submod(a, b, q) == ref.scalar_primitive("submod128", a, b, q)
         assert orc.mulmod(a, b, q, mu) == ref.scalar_primitive("mulmod128", a, b, q, mu)
+
+
+def _is_prime(n: int) -> bool:
+    """Miller-Rabin with the first 16 prime bases (deterministic far beyond
+    2^128 for composites of no special form; a check of the test moduli)."""
+    if n < 2:
+        return False
+    bases = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53)
+    for p in bases:
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d, s = d // 2, s + 1
+    for a in bases:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+# moduli of the polynomial-multiply path: q_116, the largest prime below 2^122
+# with 2^17 | q - 1, the 64-bit "Goldilocks" prime, and a tiny prime
+POLY_MODULI = {"q116": (C.Q_116, 16), "q122": ((1 << 122) - 6160383, 16),
+               "q64": ((1 << 64) - (1 << 32) + 1, 16), "q97": (97, 4)}
+
+
+@pytest.mark.parametrize("name", sorted(POLY_MODULI))
+def test_polymul_ntt_matches_schoolbook(orc, name):
+    """The oracle's NTT-based negacyclic product (used for sizes where O(N^2)
+    is too slow) equals schoolbook multiplication, for every modulus the GPU
+    tests use; the moduli are prime with the stated 2-adicity."""
+    q, lmax = POLY_MODULI[name]
+    assert _is_prime(q)
+    assert (q - 1) % (1 << (lmax + 1)) == 0
+    rng = random.Random(q)
+    for L in sorted({1, 2, 3, min(lmax, 6), min(lmax, 8)}):
+        n = 1 << L
+        a = from_ints([rng.randrange(q) for _ in range(n)])
+        b = from_ints([rng.randrange(q) for _ in range(n)])
+        assert np.array_equal(orc.polymul_ntt(a, b, q), orc.polymul_schoolbook(a, b, q))

@@ -80,3 +80,23 @@ def test_device_arith_other_modulus(chk, orc):
     assert np.array_equal(run(chk, 5, a, b, q, mu), orc.vvmul(a, b, q, mu))
     assert np.array_equal(run(chk, 0, a, b, q, mu), orc.vvadd(a, b, q))
     assert np.array_equal(run(chk, 1, a, b, q, mu), orc.vvsub(a, b, q))
+
+
+def test_montgomery_and_canonical_ops(chk):
+    """Exact Montgomery product a*w*2^-124 mod q and canonical add/sub, for the
+    negacyclic polynomial path (modulus q_116), against big integers."""
+    from benchntt_amd.residues import to_ints
+
+    q = C.Q_116
+    R = 1 << 124
+    qinv = (-pow(q, -1, R)) % R
+    rng = random.Random(21)
+    edges = [0, 1, 2, q - 1, q - 2, q // 2, (1 << 64) - 1, 1 << 64]
+    av = edges * len(edges) + [rng.randrange(q) for _ in range(20000)]
+    bv = [e for e in edges for _ in edges] + [rng.randrange(q) for _ in range(20000)]
+    a, b = from_ints(av), from_ints(bv)
+    rinv = pow(R, -1, q)
+    got = to_ints(run(chk, 7, a, b, q, qinv))
+    assert got == [x * y * rinv % q for x, y in zip(av, bv)]
+    assert to_ints(run(chk, 8, a, b, q, 0)) == [(x + y) % q for x, y in zip(av, bv)]
+    assert to_ints(run(chk, 9, a, b, q, 0)) == [(x - y) % q for x, y in zip(av, bv)]
