@@ -7,10 +7,12 @@ is one axpy call over the whole vectors with inputs resident in HBM.  `value`
 is GB/s of algorithmic traffic (48 B per element: two operands in, one out).
 
 The same JSON line also carries vvadd/vvsub/vvmul at the same size, the NTT
-rows (plain N=1024 transform per launch, uniform batch 8192 x N=1024), the
-roofline of the axpy kernel and a CPU baseline: the reference's own axpy
-(oracle/_ref, AVX-512 / AVX2 / scalar as the host supports) timed on this
-host's cores over the same vectors.
+rows (plain N=1024 transform per launch, uniform batch 8192 x N=1024 and the
+N = 16..512 sweep, the 20,000-transform uneven batch against one launch
+sequence per transform), the polynomial-multiply serving trace (saturation
+rate, p50/p99 latency over a rate sweep), the roofline of the axpy kernel and
+a CPU baseline: the reference's own axpy (oracle/_ref, AVX-512 / AVX2 / scalar
+as the host supports) timed on this host's cores over the same vectors.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -196,6 +198,114 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dic
     return out
 
 
+def dev_uniform(torch, n: int, q: int, gen):
+    """n residues uniform in [0, q) drawn on the device from a seeded torch
+    generator: a uniform 64-bit low word (torch.randint's int64 range) and a
+    high word below 2^(bitlen(q) - 64), with draws >= q redrawn."""
+    qh, ql = q >> 64, q & ((1 << 64) - 1)
+    assert ql < (1 << 63)
+    hb = q.bit_length() - 64
+    kw = dict(dtype=torch.int64, device="cuda", generator=gen)
+    lo = torch.randint(-(1 << 63), (1 << 63) - 1, (n,), **kw)
+    hi = torch.randint(0, 1 << hb, (n,), **kw)
+    while True:
+        bad = (hi > qh) | ((hi == qh) & ((lo < 0) | (lo >= ql)))
+        k = int(bad.sum())
+        if k == 0:
+            return torch.stack([lo, hi], dim=1).contiguous()
+        lo[bad] = torch.randint(-(1 << 63), (1 << 63) - 1, (k,), **kw)
+        hi[bad] = torch.randint(0, 1 << hb, (k,), **kw)
+
+
+def serving(torch, np, n_req: int = 4096, cpu: bool = True) -> dict:
+    """Negacyclic polynomial-multiply requests mod q_116 (forward NTT x2,
+    pointwise product, inverse NTT) with N uniform on 2^10..2^16, served with
+    continuous batching under a 2 GiB HBM budget on 4 streams.  First a closed
+    loop (every request arrived at t = 0) gives the saturation rate; then
+    seeded Poisson arrivals at fractions of it give p50/p99 latency (arrival to
+    completion seen by the host) and the achieved rate.  Also: the same
+    requests one at a time with no batching."""
+    from benchntt_amd import config as C
+    from benchntt_amd.polymul import PolymulServer
+
+    q = C.Q_116
+    rng = np.random.default_rng(55)
+    logs = rng.integers(10, 17, size=n_req)
+    sizes = [1 << int(l) for l in logs]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    total = int(offs[-1])
+    gen = torch.Generator(device="cuda")
+    gen.manual_seed(55)
+    A, B = dev_uniform(torch, total, q, gen), dev_uniform(torch, total, q, gen)
+    Cc = torch.empty_like(A)
+    reqs = [(A[offs[i]:offs[i + 1]], B[offs[i]:offs[i + 1]], Cc[offs[i]:offs[i + 1]]) for i in range(n_req)]
+    torch.cuda.synchronize()
+    budget = 2 << 30
+    srv = PolymulServer(q, hbm_budget=budget, max_inflight=4, max_batch=64, min_log2n=10, max_log2n=16)
+    srv.multiply(reqs[:64])  # warm-up
+    out = {"serving_requests": n_req, "serving_hbm_budget_bytes": budget,
+           "serving_mean_N": round(float(np.mean(sizes)), 1)}
+
+    def trace(arrival):
+        done, st = srv.run_trace(reqs, arrival)
+        lat = (done.astype(np.float64) - arrival.astype(np.float64)) * 1e-3  # us
+        span = float(done.max()) * 1e-9
+        return lat, span, st
+
+    zero = np.zeros(n_req, dtype=np.uint64)
+    lat, span, st = trace(zero)
+    sat = n_req / span
+    out["serving_saturation_requests_per_s"] = round(sat, 1)
+    out["serving_saturation_batches"] = int(st["batches"])
+    log(f"serving closed loop: {n_req} requests in {span * 1e3:.2f} ms -> {sat:.0f} req/s, "
+        f"{st['batches']} batches, at most {st['max_inflight_requests']} requests in flight")
+    sweep = []
+    for frac in (0.25, 0.5, 0.7, 0.85, 1.0, 1.2):
+        rate = frac * sat
+        gaps = np.random.default_rng(int(frac * 1000)).exponential(1e9 / rate, size=n_req)
+        arrival = np.cumsum(gaps).astype(np.uint64)
+        lat, span, st = trace(arrival)
+        row = {"offered_per_s": round(rate, 1), "achieved_per_s": round(n_req / span, 1),
+               "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1),
+               "batches": int(st["batches"]), "admission_waits": int(st["admission_waits"])}
+        sweep.append(row)
+        log(f"serving at {frac:.2f} x saturation: offered {rate:.0f}/s achieved {row['achieved_per_s']:.0f}/s, "
+            f"p50 {row['p50_us']} us, p99 {row['p99_us']} us, {row['batches']} batches")
+    out["serving_sweep"] = sweep
+    at = sweep[2]
+    out["serving_p50_us_at_0.7_saturation"] = at["p50_us"]
+    out["serving_p99_us_at_0.7_saturation"] = at["p99_us"]
+    # no batching: one request per bntt_pm_multiply call
+    k = 256
+    t0 = time.perf_counter()
+    for r in reqs[:k]:
+        srv.multiply([r])
+    one = (time.perf_counter() - t0) / k
+    out["serving_unbatched_requests_per_s"] = round(1 / one, 1)
+    out["serving_batched_vs_unbatched"] = round(sat * one, 2)
+    log(f"serving one request per call: {one * 1e6:.1f} us mean -> {1 / one:.0f} req/s")
+    srv.close()
+    if cpu:
+        import oracle
+
+        orc = oracle.load()
+        n = 1 << 12
+        from benchntt_amd.residues import uniform
+
+        a, b = uniform(n, q, 1), uniform(n, q, 2)
+        t0, cnt = time.perf_counter(), 0
+        while time.perf_counter() - t0 < 3.0:
+            orc.polymul_ntt(a, b, q)
+            cnt += 1
+        dt = (time.perf_counter() - t0) / cnt
+        out["serving_cpu_baseline"] = {
+            "value": round(1 / dt, 2), "unit": "requests/s at N=2^12", "cores": 1, "kind": "port",
+            "sample": f"{cnt} negacyclic products of N=2^12 mod q_116 through the exact C restatement "
+                      "(bitwise 256-by-128 reduction, table set-up included), about 3 s"}
+        log(f"serving cpu port: {1 / dt:.2f} req/s at N=2^12")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,10 +314,11 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-per-launch", action="store_true",
                     help="skip the one-launch-per-transform comparisons (keeps profiler output small)")
-    ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven"], default="all",
+    ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven", "serving"], default="all",
                     help="time one workload only and print no JSON line (for profiling runs): "
                          "axpy 2^24; copy (a 256 MiB device copy, the traffic calibration kernel); "
-                         "ntt-uniform (8192 x N=1024); uneven (20,000-transform batch)")
+                         "ntt-uniform (8192 x N=1024); uneven (20,000-transform batch); "
+                         "serving (polynomial-multiply trace)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -229,6 +340,9 @@ def main() -> int:
     props = torch.cuda.get_device_properties(0)
     log(f"device {props.name} ({getattr(props, 'gcnArchName', '?')}), {props.multi_processor_count} CUs")
 
+    if args.only == "serving":
+        serving(torch, np, cpu=False)
+        return 0
     if args.only in ("ntt-uniform", "uneven"):
         qn, mun = C.Q_NTT, C.MU_NTT
         if args.only == "uneven":
@@ -270,6 +384,7 @@ def main() -> int:
 
         extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch))
         extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch))
+        extras.update(serving(torch, np, cpu=not args.no_cpu_baseline))
 
     traffic = load_traffic("axpy_2_24")
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -568,6 +568,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
     const auto t0 = clk::now();
     auto now_ns = [&]() { return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now() - t0).count(); };
     std::deque<size_t> pending;
+    std::vector<char> waited(n, 0);
     size_t next = 0, completed = 0, inflight_reqs = 0;
     std::vector<ReqView> views;
     while (completed < n) {
@@ -601,7 +602,8 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
                 const size_t u = request_units(reqs[r].log2n);
                 size_t off;
                 if (!s->arena.alloc(u, &off)) {
-                    st.admission_waits++;
+                    if (!waited[r]) st.admission_waits++;
+                    waited[r] = 1;
                     break;
                 }
                 pending.pop_front();

@@ -191,7 +191,7 @@ typedef struct bntt_pm_stats {
     uint64_t requests;
     uint64_t batches;
     uint64_t max_inflight_requests; /* most requests on the device at once */
-    uint64_t admission_waits;       /* times an arrived request waited for budget */
+    uint64_t admission_waits;       /* requests that arrived and waited for budget */
 } bntt_pm_stats;
 
 typedef struct bntt_pm_server bntt_pm_server;

@@ -282,18 +282,23 @@ void or_intt_exact(u128 *X, const u128 *Y, int log2n, const u128 *q_, const u128
     }
 }
 
+/* Coefficients k0 <= k < k1 of the schoolbook negacyclic product
+ * c = a*b mod (x^n + 1, q):  c[k] = sum_{i<=k} a_i b_{k-i} - sum_{i>k} a_i b_{n+k-i}.
+ * Each output is independent, so callers may split [0, n) over threads. */
+void or_polymul_schoolbook_range(u128 *c, const u128 *a, const u128 *b, size_t n, const u128 *q_,
+                                 size_t k0, size_t k1)
+{
+    const u128 q = *q_;
+    for (size_t k = k0; k < k1 && k < n; k++) {
+        u128 acc = 0;
+        for (size_t i = 0; i <= k; i++) acc = addq(acc, or_mulmod_exact(a[i], b[k - i], q), q);
+        for (size_t i = k + 1; i < n; i++) acc = subq(acc, or_mulmod_exact(a[i], b[n + k - i], q), q);
+        c[k] = acc;
+    }
+}
+
 /* Schoolbook negacyclic product c = a*b mod (x^n + 1, q).  O(n^2). */
 void or_polymul_schoolbook(u128 *c, const u128 *a, const u128 *b, size_t n, const u128 *q_)
 {
-    const u128 q = *q_;
-    for (size_t k = 0; k < n; k++) c[k] = 0;
-    for (size_t i = 0; i < n; i++) {
-        if (a[i] == 0) continue;
-        for (size_t j = 0; j < n; j++) {
-            u128 p = or_mulmod_exact(a[i], b[j], q);
-            size_t k = i + j;
-            if (k < n) c[k] = addq(c[k], p, q);
-            else c[k - n] = subq(c[k - n], p, q);
-        }
-    }
+    or_polymul_schoolbook_range(c, a, b, n, q_, 0, n);
 }

@@ -86,6 +86,8 @@ class Oracle:
         L.or_ntt_exact.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _P]
         L.or_intt_exact.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _P, _P]
         L.or_polymul_schoolbook.argtypes = [_P, _P, _P, ctypes.c_size_t, _P]
+        L.or_polymul_schoolbook_range.argtypes = [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t,
+                                                  ctypes.c_size_t]
 
     # scalar primitives ------------------------------------------------------
     def addmod(self, a: int, b: int, m: int) -> int:
@@ -184,11 +186,26 @@ class Oracle:
         )
         return x
 
-    def polymul_schoolbook(self, a, b, q: int):
+    def polymul_schoolbook(self, a, b, q: int, threads: int = 1):
+        """Schoolbook negacyclic product; with threads > 1 the output
+        coefficients are split into ranges computed concurrently."""
         k = _Keep()
         a, b = (np.ascontiguousarray(v, dtype=np.uint64).reshape(-1, 2) for v in (a, b))
         c = np.empty_like(a)
-        self.lib.or_polymul_schoolbook(_ptr(c), _ptr(a), _ptr(b), a.shape[0], k(q))
+        n = a.shape[0]
+        if threads <= 1 or n < 256:
+            self.lib.or_polymul_schoolbook(_ptr(c), _ptr(a), _ptr(b), n, k(q))
+            return c
+        from concurrent.futures import ThreadPoolExecutor
+
+        qk = k(q)
+        step = -(-n // (4 * threads))
+
+        def part(k0):
+            self.lib.or_polymul_schoolbook_range(_ptr(c), _ptr(a), _ptr(b), n, qk, k0, min(n, k0 + step))
+
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(part, range(0, n, step)))
         return c
 
 

@@ -46,7 +46,8 @@ def run_batch(torch, srv, pairs):
 
 
 def expect(orc, a, b, q):
-    return orc.polymul_schoolbook(a, b, q) if a.shape[0] <= 512 else orc.polymul_ntt(a, b, q)
+    n = a.shape[0]
+    return orc.polymul_schoolbook(a, b, q, threads=16) if n <= 4096 else orc.polymul_ntt(a, b, q)
 
 
 @pytest.mark.parametrize("name", sorted(POLY_MODULI))
@@ -65,6 +66,18 @@ def test_small_sizes_schoolbook(gpu, orc, name):
     got = run_batch(gpu, srv, pairs)
     for i, ((a, b), c) in enumerate(zip(pairs, got)):
         assert np.array_equal(c, orc.polymul_schoolbook(a, b, q)), f"request {i} (N={a.shape[0]})"
+    srv.close()
+
+
+def test_schoolbook_2_10_to_2_12(gpu, orc):
+    """The serving sizes up to 2^12 (single- and two-pass transforms) over
+    q_116 against schoolbook convolution."""
+    q = C.Q_116
+    srv = server(q, min_log2n=10, max_log2n=12, max_batch=8)
+    pairs = [(rand(1 << L, q, 300 + L), rand(1 << L, q, 400 + L)) for L in (10, 11, 12)]
+    got = run_batch(gpu, srv, pairs)
+    for (a, b), c in zip(pairs, got):
+        assert np.array_equal(c, orc.polymul_schoolbook(a, b, q, threads=16)), f"N={a.shape[0]}"
     srv.close()
 
 

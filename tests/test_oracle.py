@@ -281,3 +281,11 @@ def test_polymul_ntt_matches_schoolbook(orc, name):
         a = from_ints([rng.randrange(q) for _ in range(n)])
         b = from_ints([rng.randrange(q) for _ in range(n)])
         assert np.array_equal(orc.polymul_ntt(a, b, q), orc.polymul_schoolbook(a, b, q))
+
+
+def test_schoolbook_threaded_ranges(orc):
+    """The threaded schoolbook (independent output ranges) equals the one-call
+    schoolbook."""
+    q = C.Q_116
+    a, b = uniform(1024, q, 3), uniform(1024, q, 4)
+    assert np.array_equal(orc.polymul_schoolbook(a, b, q, threads=8), orc.polymul_schoolbook(a, b, q))
