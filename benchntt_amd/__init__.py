@@ -3,7 +3,7 @@
 128-bit modular forward NTT (constant-geometry network of ntt_scalar.c:582-627)
 and vvadd / vvsub / vvmul / axpy mod q (blas_scalar.c:578-608) as hand-written
 HIP kernels behind the C ABI of include/bntt.h (libbntt.so, built in-tree by
-``python -m benchntt_amd.build``).  Importing this package loads the library
+``python benchntt_amd/build.py``).  Importing this package loads the library
 and raises ImportError when it is missing; there is no CPU fallback.
 """
 from . import config  # noqa: F401

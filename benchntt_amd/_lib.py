@@ -1,6 +1,6 @@
 """ctypes binding of libbntt.so (the C ABI of include/bntt.h).
 
-The library is built in-tree by ``python -m benchntt_amd.build``.  Loading fails
+The library is built in-tree by ``python benchntt_amd/build.py``.  Loading fails
 loudly when it is missing: there is no CPU fallback for anything here.
 """
 from __future__ import annotations
@@ -32,6 +32,10 @@ class U128(ctypes.Structure):
         if v < 0 or v >= 1 << 128:
             raise ValueError("value does not fit in 128 bits")
         return cls(v & (2**64 - 1), v >> 64)
+
+    @property
+    def value(self) -> int:
+        return self.lo | (self.hi << 64)
 
 
 class NttTask(ctypes.Structure):
@@ -115,12 +119,25 @@ _SIGS = {
     "bntt_pm_multiply": [_P, ctypes.POINTER(PmRequest), _SZ],
     "bntt_pm_run_trace": [_P, ctypes.POINTER(PmRequest), ctypes.POINTER(ctypes.c_uint64), _SZ,
                           ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(PmStats)],
+    "bntt_hilo_to_u128": [_P, _P, _SZ],
+    "bntt_u128_to_hilo": [_P, _P, _SZ],
+    "bntt_hilo_to_u128_dev": [_P, _P, _SZ, _P],
+    "bntt_u128_to_hilo_dev": [_P, _P, _SZ, _P],
+    "bntt_twiddles_dev": [_P, ctypes.c_int, _PU, ctypes.c_int, _P],
+    "bntt_twiddles": [_P, ctypes.c_int, _PU, ctypes.c_int],
+    "bntt_uniform_dev": [_P, _SZ, _PU, ctypes.c_uint64, _P],
+    "bntt_uniform": [_P, _SZ, _PU, ctypes.c_uint64],
 }
 # entry points that do not return a status
 _OTHER = {
     "bntt_last_error": ([], ctypes.c_char_p),
     "bntt_ntt_tmp_bytes": ([ctypes.c_int, _SZ], _SZ),
+    "bntt_addmod128": ([U128, U128, U128], U128),
+    "bntt_submod128": ([U128, U128, U128], U128),
+    "bntt_mulmod128": ([U128, U128, U128, U128], U128),
 }
+
+BNTT_TW_CYCLIC, BNTT_TW_NEGACYCLIC, BNTT_TW_CYCLIC_INV, BNTT_TW_NEGACYCLIC_INV = 0, 1, 2, 3
 
 EXPORTS = tuple(_SIGS) + tuple(_OTHER)
 
@@ -135,7 +152,7 @@ def load(path: str | None = None) -> ctypes.CDLL:
     p = path or LIB_PATH
     if not os.path.exists(p):
         raise ImportError(
-            f"{p} is missing: build the HIP library first (python -m benchntt_amd.build)"
+            f"{p} is missing: build the HIP library first (python benchntt_amd/build.py)"
         )
     # torch ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.  Importing
     # it first makes libbntt.so bind to those same copies (same SONAMEs), so the
@@ -147,6 +164,9 @@ def load(path: str | None = None) -> ctypes.CDLL:
     except ImportError:
         pass
     lib = ctypes.CDLL(p)
+    missing = [n for n in EXPORTS if not hasattr(lib, n)]
+    if missing:
+        raise ImportError(f"{p} lacks {', '.join(missing)}: rebuild it (python benchntt_amd/build.py)")
     for name, args in _SIGS.items():
         fn = getattr(lib, name)
         fn.argtypes = args

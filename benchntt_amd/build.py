@@ -1,6 +1,6 @@
 """Build libbntt.so (the HIP kernels and the C ABI of include/bntt.h) in-tree.
 
-    python -m benchntt_amd.build
+    python benchntt_amd/build.py
 
 Every HIP source is compiled for gfx950 only, with packed-FP32 code generation
 disabled, and linked into benchntt_amd/libbntt.so with hidden visibility except
@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbntt.so")
-SOURCES = ["api.hip", "blas.hip", "ntt.hip", "batch.hip", "serve.hip"]
+SOURCES = ["api.hip", "blas.hip", "ntt.hip", "batch.hip", "serve.hip", "io.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 CFLAGS = [

@@ -36,6 +36,7 @@
 
 #include "../../include/bntt.h"
 #include "internal.hpp"
+#include "hostmont.hpp"
 #include "queue.hpp"
 
 using bntt::DevDesc;
@@ -48,59 +49,9 @@ using bntt::WorkItem;
 namespace {
 
 typedef unsigned __int128 u128;
+using bntt::HostMont;
 
-U128 u128_to_limbs(u128 v) { return bntt::make_u128((uint64_t)v, (uint64_t)(v >> 64)); }
-u128 limbs_to_u128(const U128 &a) { return ((u128)bntt::hi64(a) << 64) | bntt::lo64(a); }
-
-// Host Montgomery arithmetic over the device formulas (compiled for the host).
-struct HostMont {
-    bntt::MontQ mq;
-    U128 r1;   // R mod q  (Montgomery form of 1)
-    R31 r2;    // R^2 mod q in digits
-    u128 q;
-
-    bool init(u128 qv)
-    {
-        q = qv;
-        if (!(q & 1) || (q >> 122) != 0 || q < 3) return false;
-        // q^-1 mod 2^128 by Newton iteration, then -q^-1 mod 2^124
-        u128 x = q;
-        for (int i = 0; i < 7; i++) x *= (u128)2 - q * x;
-        const u128 qinv = ((~x + 1) & (((u128)1 << 124) - 1));
-        mq.q = bntt::to_r31(u128_to_limbs(q));
-        mq.qinv = bntt::to_r31(u128_to_limbs(qinv));
-        mq.ql = u128_to_limbs(q);
-        // R mod q and R^2 mod q by doubling
-        u128 v = 1;
-        for (int i = 0; i < 248; i++) {
-            v <<= 1;
-            if (v >= q) v -= q;
-            if (i == 123) r1 = u128_to_limbs(v);
-        }
-        r2 = bntt::to_r31(u128_to_limbs(v));
-        return true;
-    }
-    U128 to_m(u128 x) const { return bntt::montmul_r31(u128_to_limbs(x % q), r2, mq); }
-    U128 mul(const U128 &a, const U128 &b) const { return bntt::montmul_r31(a, bntt::to_r31(b), mq); }
-    u128 from_m(const U128 &a) const
-    {
-        R31 one{};
-        one.d[0] = 1;
-        return limbs_to_u128(bntt::montmul_r31(a, one, mq));
-    }
-    U128 pow(U128 base, u128 e) const
-    {
-        U128 r = r1;
-        while (e) {
-            if (e & 1) r = mul(r, base);
-            base = mul(base, base);
-            e >>= 1;
-        }
-        return r;
-    }
-};
-
-uint32_t brv(uint32_t i, int bits) { return bits ? (__builtin_bitreverse32(i) >> (32 - bits)) : 0; }
+uint32_t brv(uint32_t i, int bits) { return bntt::brv_host(i, bits); }
 
 struct SizeTables {
     R31 *fwd = nullptr;    // tw_psi * R, digits
@@ -233,23 +184,10 @@ void destroy_server(bntt_pm_server *s)
 int build_tables(bntt_pm_server *s, int L)
 {
     const HostMont &hm = s->hm;
-    const u128 q = hm.q;
-    int v = 0;
-    while (((q - 1) >> v & 1) == 0) v++;
-    if (L + 1 > v) return bntt_set_error(BNTT_EINVAL, "q - 1 has too few factors of two for this size");
-    // generator of the 2-power part: a quadratic non-residue g, w = g^((q-1)/2^v)
-    const U128 minus_one = hm.to_m(q - 1);
     U128 w{};
-    bool found = false;
-    for (u128 g = 2; g < 1000 && !found; g++) {
-        const U128 gm = hm.to_m(g);
-        const U128 e = hm.pow(gm, (q - 1) >> 1);
-        if (std::memcmp(&e, &minus_one, sizeof e) == 0) {
-            w = hm.pow(gm, (q - 1) >> v);
-            found = true;
-        }
-    }
-    if (!found) return bntt_set_error(BNTT_EINVAL, "no quadratic non-residue below 1000 (is q prime?)");
+    int v = 0;
+    if (!hm.two_adic_root(&w, &v)) return bntt_set_error(BNTT_EINVAL, "no quadratic non-residue below 1000");
+    if (L + 1 > v) return bntt_set_error(BNTT_EINVAL, "q - 1 has too few factors of two for this size");
     const size_t n = (size_t)1 << L;
     std::vector<R31> fwd(n), inv(n), scale(n);
     fwd[0] = bntt::to_r31(hm.r1);  // entry 0 is never read
@@ -274,7 +212,7 @@ int build_tables(bntt_pm_server *s, int L)
     }
     // S[i] = N^-1 psi^-brv(i) R^2 with psi of order 2^(L+1)
     const U128 psi_inv = hm.pow(w, ((u128)1 << v) - ((u128)1 << (v - L - 1)));
-    const U128 ninv = hm.pow(hm.to_m((q + 1) / 2), (u128)L);
+    const U128 ninv = hm.pow(hm.to_m((hm.q + 1) / 2), (u128)L);
     U128 acc = ninv;  // Montgomery form of N^-1 psi^-j
     for (size_t j = 0; j < n; j++) {
         // stored = value * R^2 = montmul(value * R, R^2)
@@ -480,6 +418,10 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
     if (!s->hm.init(((u128)q->hi << 64) | q->lo)) {
         delete s;
         return bntt_set_error(BNTT_EINVAL, "q must be odd, at least 3 and below 2^122");
+    }
+    if (!s->hm.is_probable_prime()) {
+        delete s;
+        return bntt_set_error(BNTT_EINVAL, "q is not prime");
     }
     s->md.m = s->hm.mq.ql;
     s->md.mq = s->hm.mq;

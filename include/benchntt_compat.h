@@ -10,6 +10,8 @@
  *   vector_sub(c, a, b, m)                      blas_scalar.c:586 -> bntt_vvsub
  *   vector_mul(c, a, b, m, mu)                  blas_scalar.c:594 -> bntt_vvmul
  *   axpy(c, a, b, x, m, mu)                     blas_scalar.c:602 -> bntt_axpy
+ *   addmod128 / submod128 / mulmod128           ntt_scalar.c:49-199 -> bntt_addmod128 ...
+ *                                               (host; define BNTT_COMPAT_NO_PRIMITIVES to omit)
  * Sizes: the NTT size is the caller's N; the BLAS length is BNTT_BLAS_LEN,
  * 1024 by default as in the reference (blas_scalar.c:580).  Pointers are host
  * pointers and the calls are synchronous, as in the reference.  as0/as1 are
@@ -78,6 +80,27 @@ static inline int bntt_compat_log2(unsigned long n)
     while ((1UL << l) < n) l++;
     return l;
 }
+
+#ifndef BNTT_COMPAT_NO_PRIMITIVES
+/* The scalar primitives (ntt_scalar.c:49-199), bit-identical, on the host. */
+static inline uint128_t bntt_compat_v(bntt_u128 v) { return ((uint128_t)v.hi << 64) | v.lo; }
+
+static inline uint128_t addmod128(uint128_t a, uint128_t b, uint128_t m)
+{
+    return bntt_compat_v(bntt_addmod128(bntt_compat_u128(a), bntt_compat_u128(b), bntt_compat_u128(m)));
+}
+
+static inline uint128_t submod128(uint128_t a, uint128_t b, uint128_t m)
+{
+    return bntt_compat_v(bntt_submod128(bntt_compat_u128(a), bntt_compat_u128(b), bntt_compat_u128(m)));
+}
+
+static inline uint128_t mulmod128(uint128_t a, uint128_t b, uint128_t m, uint128_t mu)
+{
+    return bntt_compat_v(
+        bntt_mulmod128(bntt_compat_u128(a), bntt_compat_u128(b), bntt_compat_u128(m), bntt_compat_u128(mu)));
+}
+#endif
 
 #ifdef N
 static inline void ntt(uint128_t *Y, uint128_t *X, uint128_t modulus, uint128_t mu, uint128_t *twiddles,

@@ -208,6 +208,49 @@ int bntt_pm_multiply(bntt_pm_server *server, const bntt_pm_request *reqs, size_t
 int bntt_pm_run_trace(bntt_pm_server *server, const bntt_pm_request *reqs, const uint64_t *arrival_ns, size_t n,
                       uint64_t *done_ns, bntt_pm_stats *stats);
 
+/* ------------------------------------------------ data formats and inputs */
+
+/* The reference's scalar primitives on the host, bit-identical to
+ * addmod128 / submod128 / mulmod128 (ntt_scalar.c:49-79, 81-104, 173-199, MBITS
+ * = 124), including their edge behaviour.  Arguments by value, as there. */
+bntt_u128 bntt_addmod128(bntt_u128 a, bntt_u128 b, bntt_u128 m);
+bntt_u128 bntt_submod128(bntt_u128 a, bntt_u128 b, bntt_u128 m);
+bntt_u128 bntt_mulmod128(bntt_u128 a, bntt_u128 b, bntt_u128 m, bntt_u128 mu);
+
+/* Word order of the reference's fixture files and AVX code: residue i is
+ * hilo[2i] = high word, hilo[2i+1] = low word (config.c:6-197 reads "hi lo"
+ * pairs into this layout; ntt_avx512.c:386-528 takes it).  n residues; out may
+ * equal the input.  Host versions convert on the host; _dev versions run on
+ * `stream`. */
+int bntt_hilo_to_u128(bntt_u128 *out, const uint64_t *hilo, size_t n);
+int bntt_u128_to_hilo(uint64_t *hilo, const bntt_u128 *in, size_t n);
+int bntt_hilo_to_u128_dev(void *out, const void *hilo, size_t n, void *stream);
+int bntt_u128_to_hilo_dev(void *hilo, const void *in, size_t n, void *stream);
+
+/* Twiddle tables of roots of unity for the forward network (2^log2n entries,
+ * entry 0 = 0; stage s reads entries 2^s + k, k < 2^s):
+ *   BNTT_TW_CYCLIC      w_s^brv_s(k),        w_s of order 2^(s+1): the network
+ *                       is then the cyclic DFT in bit-reversed order;
+ *   BNTT_TW_NEGACYCLIC  p_s^(2 brv_s(k)+1),  p_s of order 2^(s+2): the
+ *                       negacyclic NTT in bit-reversed order;
+ *   *_INV               the entry-wise inverses of the same tables;
+ * with w_s, p_s powers of g^((q-1)/2^v) for the least quadratic non-residue g
+ * and 2^v || q - 1.  q must be a prime < 2^122 with v >= log2n (cyclic) or
+ * log2n + 1 (negacyclic): q_116 = 2^115 + 47*2^17 + 1 serves up to 2^16 points
+ * negacyclic.  Not possible for the reference's q_N (v = 4), whose drivers use
+ * random tables. */
+enum { BNTT_TW_CYCLIC = 0, BNTT_TW_NEGACYCLIC = 1, BNTT_TW_CYCLIC_INV = 2, BNTT_TW_NEGACYCLIC_INV = 3 };
+int bntt_twiddles_dev(void *tw, int log2n, const bntt_u128 *q, int kind, void *stream);
+int bntt_twiddles(bntt_u128 *tw, int log2n, const bntt_u128 *q, int kind);
+
+/* n residues uniform in [0, q) from seed (q >= 2): residue i keeps the first
+ * of the draws t = 0, 1, ... below q, draw t being the splitmix64 words
+ * 2(t n + i) (low) and 2(t n + i) + 1 (high), masked below 2^bitlen(q); the
+ * 64th draw, if ever reached, is kept minus q.  Deterministic per (n, q, seed)
+ * and independent of the launch shape. */
+int bntt_uniform_dev(void *out, size_t n, const bntt_u128 *q, uint64_t seed, void *stream);
+int bntt_uniform(bntt_u128 *out, size_t n, const bntt_u128 *q, uint64_t seed);
+
 /* -------------------------------------------------------------- host pointers */
 
 int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m);

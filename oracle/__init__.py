@@ -12,5 +12,8 @@ compiled by ``build_ref.sh`` into ``_ref/`` when that build exists.
 from .oracle import (  # noqa: F401
     build,
     load,
+    negacyclic_tables,
     Oracle,
+    root_table,
+    two_adic_root,
 )

@@ -302,3 +302,39 @@ void or_polymul_schoolbook(u128 *c, const u128 *a, const u128 *b, size_t n, cons
 {
     or_polymul_schoolbook_range(c, a, b, n, q_, 0, n);
 }
+
+/* ------------------------------------------------------------------------
+ * Seeded uniform residues (this build's input generator; no counterpart in
+ * the reference, whose inputs are fixed text files).  Word j of the
+ * splitmix64 sequence from `seed` (Steele, Lea and Flood, OOPSLA 2014;
+ * Vigna's public-domain splitmix64.c): z = seed + (j+1)*0x9E3779B97F4A7C15,
+ * then two xor-shift-multiply rounds and a final xor-shift.
+ */
+uint64_t or_splitmix_at(uint64_t seed, uint64_t j)
+{
+    uint64_t z = seed + (j + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+/* Residue i: the first draw t = 0, 1, ... below q of the words 2(t n + i)
+ * (low) and 2(t n + i) + 1 (high), masked below 2^bitlen(q); the 64th draw,
+ * if reached, is kept minus q. */
+void or_uniform(u128 *out, size_t n, const u128 *q_, uint64_t seed)
+{
+    const u128 q = *q_;
+    int bits = 0;
+    while (bits < 128 && (q >> bits) != 0) bits++;
+    const u128 mask = bits == 128 ? ~(u128)0 : (((u128)1 << bits) - 1);
+    for (size_t i = 0; i < n; i++) {
+        u128 v = 0;
+        for (uint64_t t = 0; t < 64; t++) {
+            const uint64_t j = 2 * (t * n + i);
+            v = (((u128)or_splitmix_at(seed, j + 1) << 64) | or_splitmix_at(seed, j)) & mask;
+            if (v < q) break;
+            if (t == 63) v -= q;
+        }
+        out[i] = v;
+    }
+}

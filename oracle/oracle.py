@@ -86,6 +86,9 @@ class Oracle:
         L.or_ntt_exact.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _P]
         L.or_intt_exact.argtypes = [_P, _P, ctypes.c_int, _P, _P, _P, _P, _P]
         L.or_polymul_schoolbook.argtypes = [_P, _P, _P, ctypes.c_size_t, _P]
+        L.or_splitmix_at.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
+        L.or_splitmix_at.restype = ctypes.c_uint64
+        L.or_uniform.argtypes = [_P, ctypes.c_size_t, _P, ctypes.c_uint64]
         L.or_polymul_schoolbook_range.argtypes = [_P, _P, _P, ctypes.c_size_t, _P, ctypes.c_size_t,
                                                   ctypes.c_size_t]
 
@@ -209,6 +212,15 @@ class Oracle:
         return c
 
 
+    def splitmix_at(self, seed: int, j: int) -> int:
+        return self.lib.or_splitmix_at(seed, j)
+
+    def uniform(self, n: int, q: int, seed: int):
+        k = _Keep()
+        out = np.empty((n, 2), dtype=np.uint64)
+        self.lib.or_uniform(_ptr(out), n, k(q), seed)
+        return out
+
     def polymul_ntt(self, a, b, q: int):
         """Negacyclic product through the exact networks: or_ntt_exact of a and
         b with the table tw[2^s + k] = psi_s^(2 brv_s(k) + 1) (psi_s of order
@@ -236,6 +248,28 @@ def two_adic_root(q: int) -> tuple[int, int]:
 
 def _brv(x: int, bits: int) -> int:
     return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def root_table(q: int, L: int, kind: str) -> list[int]:
+    """Twiddle table of 2^L entries (entry 0 = 0): 'cyclic' w_s^brv_s(k) with
+    w_s of order 2^(s+1), 'negacyclic' p_s^(2 brv_s(k)+1) with p_s of order
+    2^(s+2), '*_inv' the entry-wise inverses; the stage roots are powers of the
+    root of two_adic_root(q)."""
+    w, v = two_adic_root(q)
+    nega = kind.startswith("negacyclic")
+    inv = kind.endswith("_inv")
+    if L + (1 if nega else 0) > v:
+        raise ValueError("q - 1 has too few factors of two")
+    tw = [0] * (1 << L)
+    for s in range(L):
+        order = s + (2 if nega else 1)
+        r = pow(w, 1 << (v - order), q)
+        if inv:
+            r = pow(r, (1 << order) - 1, q)
+        for k in range(1 << s):
+            e = 2 * _brv(k, s) + 1 if nega else _brv(k, s)
+            tw[(1 << s) + k] = pow(r, e, q)
+    return tw
 
 
 def negacyclic_tables(q: int, L: int) -> tuple[list[int], list[int]]:
