@@ -101,7 +101,40 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
             "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
 
 
-def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dict:
+def cpu_threads() -> int:
+    return min(os.cpu_count() or 1, 16)
+
+
+def ntt_cpu_rows(np, x, tw, qn, mun) -> dict:
+    """The reference's ntt() at N = 1024 on this host: one core, 50 timed
+    calls after 51 warm-up calls (the reference times 100 calls and averages
+    the last 50, ntt_scalar.c:670-683), scalar and the widest AVX build; then
+    the uniform batch rate with one transform stream per thread."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import ref
+
+    if not ref.available("ref_ntt_scalar_L10.so"):
+        return {}
+    out = {"ntt_cpu_model": ref.cpu_model()}
+    isas = ["scalar"] + ([ref.best_isa()] if ref.best_isa() != "scalar" else [])
+    for isa in isas:
+        ref.ntt(x, tw, qn, mun, isa=isa, iters=50)
+        _, ns = ref.ntt(x, tw, qn, mun, isa=isa, iters=50)
+        out[f"ntt_1024_cpu_{isa}_ns_per_butterfly_1core"] = round(ns / 5120, 3)
+    isa = isas[-1]
+    T = cpu_threads()
+    with ThreadPoolExecutor(T) as ex:
+        ns = list(ex.map(lambda _: ref.ntt(x, tw, qn, mun, isa=isa, iters=256)[1], range(T)))
+    out[f"ntt_uniform_1024_cpu_{isa}_NTTs_per_s"] = round(T / (float(np.mean(ns)) * 1e-9), 1)
+    out["ntt_cpu_threads"] = T
+    log(f"cpu ntt 1024: " + ", ".join(f"{k.split('_')[3]} {v} ns/butterfly" for k, v in out.items()
+                                      if k.endswith("1core")) + f"; {T} threads {isa}: "
+        f"{out[f'ntt_uniform_1024_cpu_{isa}_NTTs_per_s']:.0f} NTT/s")
+    return out
+
+
+def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: bool = False) -> dict:
     """Plain transform (N=1024, one launch) and uniform batches: 8192 x N=1024
     and N in {16..512} with 2^23 coefficients, one shared seeded twiddle table
     per N, as one persistent work-queue execution (graph replay) against one
@@ -111,8 +144,9 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dic
 
     dev = torch.device("cuda:0")
     out = {}
-    tw = torch.from_numpy(uniform(1024, qn, 4).view(np.int64)).to(dev)
-    x1 = torch.from_numpy(uniform(1024, qn, 3).view(np.int64)).to(dev)
+    tw_np, x_np = uniform(1024, qn, 4), uniform(1024, qn, 3)
+    tw = torch.from_numpy(tw_np.view(np.int64)).to(dev)
+    x1 = torch.from_numpy(x_np.view(np.int64)).to(dev)
     y1 = torch.empty_like(x1)
     ms1 = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), 200, 20, stream)
     out["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
@@ -145,10 +179,37 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dic
         log(f"uniform {cnt}x{n}: persistent {msp:.4f} ms ({out[f'{key}_NTTs_per_s']:.0f} NTT/s)"
             + (f", one launch {msd:.4f} ms, per-transform launches {mse:.2f} ms" if L == 10 else ""))
     log(f"ntt plain 1024: {ms1 * 1e3:.2f} us")
+    if cpu:
+        out.update(ntt_cpu_rows(np, x_np, tw_np, qn, mun))
     return out
 
 
-def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dict:
+def uneven_cpu(np, wl, qn, mun) -> dict:
+    """The same 20,000 transforms through the reference's scalar ntt() (one
+    build per size), spread over the host's threads: NTTs/s = transforms /
+    (sum of the per-transform times, measured concurrently, / threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import ref
+
+    if not all(ref.available(f"ref_ntt_scalar_L{l}.so") for l in set(int(v) for v in wl.log2n)):
+        return {}
+    T = cpu_threads()
+
+    def one(i):
+        l, off = int(wl.log2n[i]), int(wl.offsets[i])
+        return ref.ntt(wl.x[off:off + (1 << l)], wl.tables[l], qn, mun, isa="scalar", iters=1)[1]
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(T) as ex:
+        ns = float(sum(ex.map(one, range(len(wl.log2n)))))
+    wall = time.perf_counter() - t0
+    rate = len(wl.log2n) / (ns * 1e-9 / T)
+    log(f"cpu uneven 20000 (reference scalar, {T} threads): {rate:.0f} NTT/s, wall {wall:.1f} s")
+    return {"uneven_cpu_scalar_NTTs_per_s": round(rate, 1), "uneven_cpu_threads": T}
+
+
+def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: bool = False) -> dict:
     """20,000 forward NTTs, N log-uniform on 2^4..2^16 in pairs (seeded), one
     twiddle table per size, each pair's pointwise product after its two
     transforms: one persistent work-queue kernel per level, replayed from a
@@ -195,6 +256,9 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True) -> dic
     log(f"uneven 20000: batched {ms_only:.3f} ms ({out['uneven_batched_NTTs_per_s']:.0f} NTT/s), "
         f"with products {ms_full:.3f} ms, per-launch {ms_each:.1f} ms -> "
         f"{out['uneven_batched_vs_per_launch']}x")
+    if cpu:
+        del dx, dy, dc, tasks, muls
+        out.update(uneven_cpu(np, wl, qn, mun))
     return out
 
 
@@ -382,8 +446,9 @@ def main() -> int:
             extras[f"{name}_GBps"] = round(n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9, 1)
             log(f"{name} 2^24: {ms * 1e3:.1f} us/call, {extras[f'{name}_GBps']} GB/s")
 
-        extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch))
-        extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch))
+        cpu = not args.no_cpu_baseline
+        extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
+        extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
         extras.update(serving(torch, np, cpu=not args.no_cpu_baseline))
 
     traffic = load_traffic("axpy_2_24")
