@@ -101,6 +101,38 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
             "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
 
 
+# VALU instructions per butterfly of the radix-8 register network of the NTT
+# kernels (reg_network_fast<3>: 12 butterflies in 1325 VALU instructions,
+# counted in the gfx950 ISA of a kernel that loads 8 values and 7 twiddles,
+# runs the network and stores; the network's product is 20 v_mad_u64_u32)
+VALU_PER_BUTTERFLY = 110
+
+
+def valu_rates() -> dict:
+    """Measured issue rates of the integer VALU instructions (library probe)."""
+    import ctypes
+
+    from benchntt_amd import _lib
+
+    out = {}
+    for name, op in (("v_mad_u64_u32", 0), ("v_add_co_u32", 1), ("v_cndmask_b32", 2), ("v_lshl_add_u64", 3),
+                     ("v_and_b32", 4)):
+        r, nom = ctypes.c_double(), ctypes.c_double()
+        _lib.call("bntt_probe_valu_rate", op, ctypes.byref(r), ctypes.byref(nom))
+        out[name] = r.value
+        out["nominal"] = nom.value
+    log("VALU issue rates (T lane-ops/s): " + ", ".join(f"{k} {v / 1e12:.2f}" for k, v in out.items()))
+    return out
+
+
+def ntt_roofline(gbfly_per_s: float, rates: dict, what: str) -> dict:
+    peak = rates["v_mad_u64_u32"]
+    ach = gbfly_per_s * 1e9 * VALU_PER_BUTTERFLY
+    return {"bound": "int-valu", "kernel": what, "achieved": round(ach / 1e12, 2), "peak": round(peak / 1e12, 2),
+            "unit": "T lane-ops/s", "frac": round(ach / peak, 4), "valu_per_butterfly": VALU_PER_BUTTERFLY,
+            "nominal_peak": round(rates["nominal"] / 1e12, 2)}
+
+
 def cpu_threads() -> int:
     return min(os.cpu_count() or 1, 16)
 
@@ -262,25 +294,6 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     return out
 
 
-def dev_uniform(torch, n: int, q: int, gen):
-    """n residues uniform in [0, q) drawn on the device from a seeded torch
-    generator: a uniform 64-bit low word (torch.randint's int64 range) and a
-    high word below 2^(bitlen(q) - 64), with draws >= q redrawn."""
-    qh, ql = q >> 64, q & ((1 << 64) - 1)
-    assert ql < (1 << 63)
-    hb = q.bit_length() - 64
-    kw = dict(dtype=torch.int64, device="cuda", generator=gen)
-    lo = torch.randint(-(1 << 63), (1 << 63) - 1, (n,), **kw)
-    hi = torch.randint(0, 1 << hb, (n,), **kw)
-    while True:
-        bad = (hi > qh) | ((hi == qh) & ((lo < 0) | (lo >= ql)))
-        k = int(bad.sum())
-        if k == 0:
-            return torch.stack([lo, hi], dim=1).contiguous()
-        lo[bad] = torch.randint(-(1 << 63), (1 << 63) - 1, (k,), **kw)
-        hi[bad] = torch.randint(0, 1 << hb, (k,), **kw)
-
-
 def serving(torch, np, n_req: int = 4096, cpu: bool = True) -> dict:
     """Negacyclic polynomial-multiply requests mod q_116 (forward NTT x2,
     pointwise product, inverse NTT) with N uniform on 2^10..2^16, served with
@@ -288,8 +301,10 @@ def serving(torch, np, n_req: int = 4096, cpu: bool = True) -> dict:
     loop (every request arrived at t = 0) gives the saturation rate; then
     seeded Poisson arrivals at fractions of it give p50/p99 latency (arrival to
     completion seen by the host) and the achieved rate.  Also: the same
-    requests one at a time with no batching."""
+    requests one at a time with no batching.  Operands: seeded uniform
+    residues from the library's device generator (bntt_uniform_dev)."""
     from benchntt_amd import config as C
+    from benchntt_amd import fixtures as fx
     from benchntt_amd.polymul import PolymulServer
 
     q = C.Q_116
@@ -298,9 +313,7 @@ def serving(torch, np, n_req: int = 4096, cpu: bool = True) -> dict:
     sizes = [1 << int(l) for l in logs]
     offs = np.concatenate([[0], np.cumsum(sizes)])
     total = int(offs[-1])
-    gen = torch.Generator(device="cuda")
-    gen.manual_seed(55)
-    A, B = dev_uniform(torch, total, q, gen), dev_uniform(torch, total, q, gen)
+    A, B = fx.uniform(total, q, 551), fx.uniform(total, q, 552)
     Cc = torch.empty_like(A)
     reqs = [(A[offs[i]:offs[i + 1]], B[offs[i]:offs[i + 1]], Cc[offs[i]:offs[i + 1]]) for i in range(n_req)]
     torch.cuda.synchronize()
@@ -376,6 +389,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--skip-serving", action="store_true", help="leave out the serving trace")
     ap.add_argument("--skip-per-launch", action="store_true",
                     help="skip the one-launch-per-transform comparisons (keeps profiler output small)")
     ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven", "serving"], default="all",
@@ -449,7 +463,15 @@ def main() -> int:
         cpu = not args.no_cpu_baseline
         extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
         extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
-        extras.update(serving(torch, np, cpu=not args.no_cpu_baseline))
+        if not args.skip_serving:
+            extras.update(serving(torch, np, cpu=not args.no_cpu_baseline))
+        rates = valu_rates()
+        extras["valu_issue_rates_T_lane_ops_per_s"] = {k: round(v / 1e12, 3) for k, v in rates.items()}
+        extras["ntt_roofline_uniform_8192x1024"] = ntt_roofline(
+            extras["ntt_uniform_8192x1024_single_launch_NTTs_per_s"] * 5120 / 1e9, rates,
+            "single-launch kernel, 8192 x N=1024 (ntt_pass_kernel<3>)")
+        extras["ntt_roofline_uneven_20000"] = ntt_roofline(
+            extras["uneven_Gbutterflies_per_s"], rates, "persistent work-queue kernel, 20,000 transforms 2^4..2^16")
 
     traffic = load_traffic("axpy_2_24")
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -127,6 +127,7 @@ _SIGS = {
     "bntt_twiddles": [_P, ctypes.c_int, _PU, ctypes.c_int],
     "bntt_uniform_dev": [_P, _SZ, _PU, ctypes.c_uint64, _P],
     "bntt_uniform": [_P, _SZ, _PU, ctypes.c_uint64],
+    "bntt_probe_valu_rate": [ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)],
 }
 # entry points that do not return a status
 _OTHER = {

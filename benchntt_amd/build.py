@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbntt.so")
-SOURCES = ["api.hip", "blas.hip", "ntt.hip", "batch.hip", "serve.hip", "io.hip"]
+SOURCES = ["api.hip", "blas.hip", "ntt.hip", "batch.hip", "serve.hip", "io.hip", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 CFLAGS = [

@@ -10,16 +10,18 @@
 // (1/2/4/8), block size (128..1024), one-shot versus grid-stride grids and
 // temporal versus non-temporal accesses: one element per lane in 512-thread
 // blocks with non-temporal loads and ordinary (write-back) stores moved about
-// 7.0 TB/s for both vvadd and axpy, against 6.5 TB/s with non-temporal stores
-// and 5.8 TB/s with ordinary loads.  vvmul/axpy add one mulmod per element
-// (about 75 VALU ops), which the memory time hides.
+// 7.0 TB/s for vvadd, against 6.5 TB/s with non-temporal stores and 5.8 TB/s
+// with ordinary loads.  vvmul/axpy add one mulmod per element (about 75 VALU
+// ops, 68 VGPRs); with the wave-uniform fast path, axpy moved 6.7-6.9 TB/s in
+// 512-thread blocks and 7.0 TB/s in 256-thread blocks (1024: 6.7 TB/s; two
+// or four elements per lane: 6.8-6.9 TB/s), so all four use 256.
 #include "kernels.hpp"
 
 namespace bntt {
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
-constexpr int kBlasThreads = 512;
+constexpr int kBlasThreads = 256;
 constexpr int kPerLane = 1;
 
 __device__ __forceinline__ U128 load_nt(const U128 *p)

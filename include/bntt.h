@@ -251,6 +251,21 @@ int bntt_twiddles(bntt_u128 *tw, int log2n, const bntt_u128 *q, int kind);
 int bntt_uniform_dev(void *out, size_t n, const bntt_u128 *q, uint64_t seed, void *stream);
 int bntt_uniform(bntt_u128 *out, size_t n, const bntt_u128 *q, uint64_t seed);
 
+/* ------------------------------------------------------------ diagnostics */
+/* No counterpart in the reference.  Measured issue rate of one integer VALU
+ * instruction on the current device, in lane-instructions per second (8
+ * independent chains per lane, 16 workgroups of 256 threads per CU), and the
+ * nominal one-per-lane-per-clock rate CUs * 64 * clock.  The NTT's roofline
+ * is this rate divided by its VALU instructions per butterfly. */
+enum {
+    BNTT_PROBE_MAD_U64_U32 = 0,
+    BNTT_PROBE_ADD_CO_U32 = 1,
+    BNTT_PROBE_CNDMASK_B32 = 2,
+    BNTT_PROBE_LSHL_ADD_U64 = 3,
+    BNTT_PROBE_AND_B32 = 4
+};
+int bntt_probe_valu_rate(int op, double *lane_ops_per_s, double *nominal_lane_ops_per_s);
+
 /* -------------------------------------------------------------- host pointers */
 
 int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m);
