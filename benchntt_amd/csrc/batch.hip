@@ -169,7 +169,7 @@ int launch_levels(bntt_batch_plan *p, hipStream_t s)
         if (p->level_items[l] == 0) continue;
         hipLaunchKernelGGL(bntt::queue_kernel<false>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
                            p->level_lds[l], s, p->items_d + p->level_offset[l], (uint32_t)p->level_items[l],
-                           p->descs_d, p->muls_d, nullptr, p->ctl_d, p->md);
+                           p->descs_d, p->muls_d, p->ctl_d, p->md);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "queue kernel launch");
     }

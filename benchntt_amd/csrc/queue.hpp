@@ -9,8 +9,10 @@
 //   ITEM_NTT   stages [S, S+k) of the forward network for G groups (several
 //              whole transforms when k = log2 N, else blocks of one transform)
 //   ITEM_MUL   2048 elements of c = a * b mod q
-//   ITEM_GMUL  2048 elements of G[i] = A[brv(i)] * B[brv(i)] (Montgomery)
-//   ITEM_SCAT  2048 elements of C[brv(i)] = Y[i] * S[i] (Montgomery)
+// With Montgomery arithmetic a transform item may also fuse the pointwise
+// steps of a negacyclic product into its first load and its last store:
+//   IT_GATHER_MUL     the first pass reads x[i] = A[brv(i)] * B[brv(i)] * R^-1
+//   IT_SCATTER_SCALE  the last pass writes C[brv(i)] = y[i] * S[i] * R^-1
 // where brv reverses the log2 N index bits of a transform.
 #pragma once
 
@@ -24,7 +26,8 @@ constexpr int kQueueTile = kQueueThreads << kQueueMaxElog;  // 2048 values
 // padded data tile (the twiddles are read from device memory)
 constexpr size_t kQueueMaxLds = 64 * 1024;
 
-enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1, ITEM_GMUL = 2, ITEM_SCAT = 3 };
+enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
+enum ItemFlags : uint8_t { IT_GATHER_MUL = 1, IT_SCATTER_SCALE = 2 };
 
 struct DevDesc {
     const U128 *x;
@@ -33,6 +36,8 @@ struct DevDesc {
     const U128 *tw;
     const R31 *twr;          // digit copy of tw (reference arithmetic) or tw*R digits (Montgomery)
     const uint32_t *twflag;  // nonzero when some entry of tw (past entry 0) is not small
+    const U128 *x2;          // IT_GATHER_MUL: second factor B (x holds A)
+    const R31 *scale;        // IT_SCATTER_SCALE: S in digits
 };
 
 struct DevMul {
@@ -42,23 +47,14 @@ struct DevMul {
     uint64_t n;
 };
 
-// Operands of a permuted pointwise product (ITEM_GMUL / ITEM_SCAT).
-struct DevPoint {
-    const U128 *src0;
-    const U128 *src1;  // GMUL: second operand
-    const R31 *tab;    // SCAT: per-index factor in digits
-    U128 *dst;
-    uint32_t log2n;
-    uint32_t pad;
-};
-
 struct WorkItem {
     uint8_t kind, L, S, k;
     uint8_t Glog, elog, pass, np;
     uint32_t desc;    // descriptor / product / point index
     uint32_t nvalid;  // groups (NTT) or elements (products)
     uint64_t first;   // first group of the pass (NTT multi-pass) or element offset (products)
-    uint64_t pad;
+    uint8_t flags;    // ItemFlags (Montgomery transform items)
+    uint8_t pad[7];
 };
 static_assert(sizeof(WorkItem) == 32, "work item layout");
 
@@ -96,9 +92,14 @@ __device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, in
     }
 }
 
+__device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
+{
+    return log2n ? (__brev(i) >> (32u - log2n)) : 0u;
+}
+
 // One transform item: load, network, store.
 template <bool MONT>
-__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkItem &it,
+__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, const WorkItem &it,
                             const DevDesc *__restrict__ descs, const Mod &md)
 {
     const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
@@ -117,11 +118,16 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
         dst = ((np - 1 - p) % 2 == 0) ? d0.y : d0.tmp;
         src = (p == 0) ? d0.x : (((np - p) % 2 == 0) ? d0.y : d0.tmp);
     }
+    // fused pointwise steps (Montgomery products only)
+    const bool gather = MONT && (it.flags & IT_GATHER_MUL) && it.pass == 0;
+    const bool scatter = MONT && (it.flags & IT_SCATTER_SCALE) && it.pass + 1 == it.np;
     if (mlog == 0) {
         // whole transforms, one per group, each with its own buffers
         if ((int)threadIdx.x < nvalid) {
-            s_x[threadIdx.x] = descs[it.desc + threadIdx.x].x;
-            s_y[threadIdx.x] = descs[it.desc + threadIdx.x].y;
+            const DevDesc &d = descs[it.desc + threadIdx.x];
+            s_x[threadIdx.x] = d.x;
+            s_y[threadIdx.x] = d.y;
+            if constexpr (MONT) s_x2[threadIdx.x] = d.x2;
         }
         __syncthreads();
     }
@@ -131,7 +137,13 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
         if (mlog == 0) {
             const int g = e >> k, ti = e & ((1 << k) - 1);
             if (g < nvalid) {
-                const U128 a = ld16(s_x[g] + ti);
+                U128 a;
+                if (gather) {
+                    const uint32_t r = bitrev((uint32_t)ti, (uint32_t)L);
+                    a = montmul_r31(ld16(s_x[g] + r), to_r31(ld16(s_x2[g] + r)), md.mq);
+                } else {
+                    a = ld16(s_x[g] + ti);
+                }
                 big |= is_big(a);
                 lds[lidx(e, k)] = a;
             }
@@ -139,7 +151,14 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
             const int g = e & (G - 1), ti = e >> Glog;
             if (g < nvalid) {
                 const size_t M = it.first + (size_t)g;
-                const U128 a = ld16(src + ((size_t)ti << mlog) + M);
+                const size_t idx = ((size_t)ti << mlog) + M;
+                U128 a;
+                if (gather) {
+                    const uint32_t r = bitrev((uint32_t)idx, (uint32_t)L);
+                    a = montmul_r31(ld16(d0.x + r), to_r31(ld16(d0.x2 + r)), md.mq);
+                } else {
+                    a = ld16(src + idx);
+                }
                 big |= is_big(a);
                 lds[lidx((g << k) + ti, k)] = a;
             }
@@ -176,9 +195,21 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const WorkI
         const int e = threadIdx.x + j * kQueueThreads;
         if (mlog == 0) {
             const int g = e >> k;
-            if (g < nvalid) st16(s_y[g] + (e & ((1 << k) - 1)), lds[lidx(e, k)]);
+            if (g < nvalid) {
+                const uint32_t ti = (uint32_t)(e & ((1 << k) - 1));
+                if (scatter)
+                    st16(s_y[g] + bitrev(ti, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[ti], md.mq));
+                else
+                    st16(s_y[g] + ti, lds[lidx(e, k)]);
+            }
         } else {
-            if ((e >> k) < nvalid) st16(dst + (it.first << k) + e, lds[lidx(e, k)]);
+            if ((e >> k) < nvalid) {
+                const size_t idx = (it.first << k) + (size_t)e;
+                if (scatter)
+                    st16(dst + bitrev((uint32_t)idx, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[idx], md.mq));
+                else
+                    st16(dst + idx, lds[lidx(e, k)]);
+            }
         }
     }
 }
@@ -197,44 +228,17 @@ __device__ __forceinline__ void do_mul_item(const WorkItem &it, const DevMul *__
     }
 }
 
-__device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
-{
-    return log2n ? (__brev(i) >> (32u - log2n)) : 0u;
-}
-
-// G[i] = A[brv(i)] * B[brv(i)] * R^-1  or  C[brv(i)] = Y[i] * S[i] * R^-1.
-template <int KIND>
-__device__ __forceinline__ void do_point_item(const WorkItem &it, const DevPoint *__restrict__ pts, const Mod &md)
-{
-    const DevPoint pt = pts[it.desc];
-#pragma unroll
-    for (int j = 0; j < kQueueTile / kQueueThreads; j++) {
-        const uint32_t e = threadIdx.x + j * kQueueThreads;
-        if (e < it.nvalid) {
-            const uint32_t i = (uint32_t)it.first + e;
-            const uint32_t r = bitrev(i, pt.log2n);
-            if constexpr (KIND == ITEM_GMUL) {
-                const U128 a = ld16(pt.src0 + r), b = ld16(pt.src1 + r);
-                st16(pt.dst + i, montmul_r31(a, to_r31(b), md.mq));
-            } else {
-                const U128 y = ld16(pt.src0 + i);
-                st16(pt.dst + r, montmul_r31(y, pt.tab[i], md.mq));
-            }
-        }
-    }
-}
-
 template <bool MONT>
 __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
                                                                  const DevDesc *__restrict__ descs,
-                                                                 const DevMul *__restrict__ muls,
-                                                                 const DevPoint *__restrict__ pts, QueueCtl *ctl,
+                                                                 const DevMul *__restrict__ muls, QueueCtl *ctl,
                                                                  Mod md)
 {
     extern __shared__ U128 lds[];
     __shared__ uint32_t s_ticket;
     __shared__ const U128 *s_x[kQueueThreads];  // buffers of the transforms of a packed item
     __shared__ U128 *s_y[kQueueThreads];
+    __shared__ const U128 *s_x2[MONT ? kQueueThreads : 1];
     for (;;) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->ticket, 1u);
         __syncthreads();
@@ -243,13 +247,8 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
         if (t >= nitems) break;
         const WorkItem it = items[t];
         if (it.kind == ITEM_NTT) {
-            do_ntt_item<MONT>(lds, s_x, s_y, it, descs, md);
-        } else if constexpr (MONT) {
-            if (it.kind == ITEM_GMUL)
-                do_point_item<ITEM_GMUL>(it, pts, md);
-            else
-                do_point_item<ITEM_SCAT>(it, pts, md);
-        } else {
+            do_ntt_item<MONT>(lds, s_x, s_y, s_x2, it, descs, md);
+        } else if constexpr (!MONT) {
             do_mul_item(it, muls, md);
         }
     }

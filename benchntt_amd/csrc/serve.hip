@@ -5,9 +5,9 @@
 // Pipeline of one request (N = 2^L, q odd < 2^122 with 2^(L+1) | q - 1), all
 // products exact Montgomery products (R = 2^124) so no value leaves [0, q):
 //   FA = F(a; psi table), FB = F(b; psi table)      F = the forward network
-//   G[k] = FA[brv k] * FB[brv k] * R^-1               (ITEM_GMUL)
+//   G[k] = FA[brv k] * FB[brv k] * R^-1               (fused into the first load of Y)
 //   Y    = F(G; inverse-root table)
-//   c[brv i] = Y[i] * S[i] * R^-1                     (ITEM_SCAT)
+//   c[brv i] = Y[i] * S[i] * R^-1                     (fused into the last store of Y)
 // with tw_psi[2^s + brv_s(j)] = psi_{2^(s+2)}^(2j+1) (the network is then the
 // negacyclic NTT in bit-reversed order), tw_inv[2^s + brv_s(j)] =
 // omega_{2^(s+1)}^-j (the cyclic inverse DFT in bit-reversed order), and
@@ -18,12 +18,14 @@
 // montmul(v, tw*R) = v*tw stays in the ordinary domain; the R^-1 of the
 // pointwise product is cancelled by the extra R in S.
 //
-// A request needs 4N residues of intermediates (FA, FB, G, Y; multi-pass
-// transforms borrow the buffer that is idle at the time), allocated from an
-// arena of a fixed HBM budget.  The trace runner admits arrived requests in
-// order while the arena has room, launches each batch as six persistent
-// work-queue launches on one of several streams, and retires batches as their
-// events complete, so batches of different arrivals overlap on the device.
+// A request needs 2N residues of intermediates (FA, FB) when its transforms
+// take one pass, 4N (FA, FB and one pass buffer for each of a and b; the
+// inverse reuses a's) when they take two, allocated from an arena of a fixed
+// HBM budget.  The trace runner admits arrived requests in order while the
+// arena has room, launches each batch as four persistent work-queue launches
+// (forward pass 0 / 1, inverse pass 0 / 1) on one of several streams, and
+// retires batches as their events complete, so batches of different arrivals
+// overlap on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,7 +42,6 @@
 #include "queue.hpp"
 
 using bntt::DevDesc;
-using bntt::DevPoint;
 using bntt::Mod;
 using bntt::R31;
 using bntt::U128;
@@ -112,13 +113,17 @@ constexpr size_t kUnit = 256;
 int passes_for(int L) { return L <= 11 ? 1 : (L + 7) / 8; }
 int elog_for(int L) { return L < 3 ? L : 3; }
 
-size_t request_units(int L) { return ((((size_t)4) << L) * sizeof(U128) + kUnit - 1) / kUnit; }
+size_t request_units(int L)
+{
+    const size_t bufs = passes_for(L) == 1 ? 2 : 4;
+    return ((bufs << L) * sizeof(U128) + kUnit - 1) / kUnit;
+}
 
-// Upper bound of work items one request contributes.
+// Upper bound of work items one request contributes (three transforms).
 size_t items_per_request(int L)
 {
     const int np = passes_for(L);
-    if (np == 1) return 3 + 2 * (((size_t)1 << L) + bntt::kQueueTile - 1) / bntt::kQueueTile;
+    if (np == 1) return 3;
     size_t per_transform = 0;
     for (int p = 0, S = 0; p < np; p++) {
         const int k = (L - S) / (np - p);
@@ -126,7 +131,7 @@ size_t items_per_request(int L)
         per_transform += (((size_t)1 << (L - k)) + ((size_t)1 << Glog) - 1) >> Glog;
         S += k;
     }
-    return 3 * per_transform + 2 * (((size_t)1 << L) + bntt::kQueueTile - 1) / bntt::kQueueTile;
+    return 3 * per_transform;
 }
 
 struct Slot {
@@ -135,9 +140,8 @@ struct Slot {
     bntt::QueueCtl *ctl = nullptr;
     WorkItem *items_d = nullptr;
     DevDesc *descs_d = nullptr;
-    DevPoint *pts_d = nullptr;
-    char *stage_h = nullptr;  // pinned staging: items | descs | points
-    size_t cap_items = 0, cap_descs = 0, cap_pts = 0;
+    char *stage_h = nullptr;  // pinned staging: items | descs
+    size_t cap_items = 0, cap_descs = 0;
     bool busy = false;
     std::vector<size_t> reqs;                          // request indices of the batch
     std::vector<std::pair<size_t, size_t>> allocs;     // arena blocks
@@ -174,7 +178,6 @@ void destroy_server(bntt_pm_server *s)
         (void)hipFree(sl.ctl);
         (void)hipFree(sl.items_d);
         (void)hipFree(sl.descs_d);
-        (void)hipFree(sl.pts_d);
         if (sl.stage_h) (void)hipHostFree(sl.stage_h);
     }
     (void)hipFree(s->arena_d);
@@ -231,20 +234,17 @@ int build_tables(bntt_pm_server *s, int L)
     return BNTT_OK;
 }
 
-int make_slot(Slot &sl, size_t cap_items, size_t cap_descs, size_t cap_pts)
+int make_slot(Slot &sl, size_t cap_items, size_t cap_descs)
 {
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
         return bntt_set_error(BNTT_EHIP, "creating a serving stream failed");
     sl.cap_items = cap_items;
     sl.cap_descs = cap_descs;
-    sl.cap_pts = cap_pts;
     if (hipMalloc(&sl.ctl, sizeof(bntt::QueueCtl)) != hipSuccess ||
         hipMalloc(&sl.items_d, cap_items * sizeof(WorkItem)) != hipSuccess ||
         hipMalloc(&sl.descs_d, cap_descs * sizeof(DevDesc)) != hipSuccess ||
-        hipMalloc(&sl.pts_d, cap_pts * sizeof(DevPoint)) != hipSuccess ||
-        hipHostMalloc(&sl.stage_h,
-                      cap_items * sizeof(WorkItem) + cap_descs * sizeof(DevDesc) + cap_pts * sizeof(DevPoint),
+        hipHostMalloc(&sl.stage_h, cap_items * sizeof(WorkItem) + cap_descs * sizeof(DevDesc),
                       hipHostMallocDefault) != hipSuccess)
         return bntt_set_error(BNTT_ENOMEM, "allocating serving buffers failed");
     if (hipMemset(sl.ctl, 0, sizeof(bntt::QueueCtl)) != hipSuccess)
@@ -261,7 +261,7 @@ struct ReqView {
 
 // Transforms of one size sharing one table: packed several per item when single-pass.
 void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector<DevDesc> &descs,
-                    const std::vector<DevDesc> &group, int L)
+                    const std::vector<DevDesc> &group, int L, uint8_t flags)
 {
     const int np = passes_for(L), elog = elog_for(L);
     if (np == 1) {
@@ -273,6 +273,7 @@ void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector
             w.Glog = (uint8_t)Glog;
             w.elog = (uint8_t)elog;
             w.np = 1;
+            w.flags = flags;
             w.desc = (uint32_t)descs.size();
             w.nvalid = (uint32_t)std::min(group.size() - i, (size_t)1 << Glog);
             for (size_t j = i; j < i + w.nvalid; j++) descs.push_back(group[j]);
@@ -297,6 +298,7 @@ void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector
                 w.elog = (uint8_t)elog;
                 w.pass = (uint8_t)p;
                 w.np = (uint8_t)np;
+                w.flags = flags;
                 w.desc = di;
                 w.nvalid = (uint32_t)std::min(groups - g0, (size_t)1 << Glog);
                 w.first = g0;
@@ -307,78 +309,61 @@ void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector
     }
 }
 
-void add_points(std::vector<WorkItem> &lv, std::vector<DevPoint> &pts, const DevPoint &pt, uint8_t kind)
-{
-    const uint32_t pi = (uint32_t)pts.size();
-    pts.push_back(pt);
-    const size_t n = (size_t)1 << pt.log2n;
-    for (size_t off = 0; off < n; off += bntt::kQueueTile) {
-        WorkItem w{};
-        w.kind = kind;
-        w.desc = pi;
-        w.nvalid = (uint32_t)std::min<size_t>(n - off, bntt::kQueueTile);
-        w.first = off;
-        lv.push_back(w);
-    }
-}
-
-// Build the six levels of a batch into the slot's staging buffer and launch them.
+// Build the four levels of a batch into the slot's staging buffer and launch them.
 int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
 {
-    // levels: 0,1 forward passes; 2 gather-multiply; 3,4 inverse passes; 5 scale-scatter
-    std::vector<std::vector<WorkItem>> lv(6);
+    // levels: 0, 1 forward passes; 2, 3 inverse passes (single-pass sizes use 0 and 2)
+    constexpr int kLevels = 4;
+    std::vector<std::vector<WorkItem>> lv(kLevels);
     std::vector<DevDesc> descs;
-    std::vector<DevPoint> pts;
-    const U128 *arena = reinterpret_cast<const U128 *>(s->arena_d);
+    U128 *arena = reinterpret_cast<U128 *>(s->arena_d);
     for (int L = s->min_l; L <= s->max_l; L++) {
         std::vector<DevDesc> fwd, inv;
         for (const ReqView &r : reqs) {
             if (r.L != L) continue;
             const size_t n = (size_t)1 << L;
-            U128 *base = const_cast<U128 *>(arena) + r.arena_off * (kUnit / sizeof(U128));
-            U128 *FA = base, *FB = base + n, *G = base + 2 * n, *Y = base + 3 * n;
-            fwd.push_back(DevDesc{r.a, FA, Y, nullptr, s->tabs[L].fwd, nullptr});
-            fwd.push_back(DevDesc{r.b, FB, G, nullptr, s->tabs[L].fwd, nullptr});
-            inv.push_back(DevDesc{G, Y, FA, nullptr, s->tabs[L].inv, nullptr});
-            add_points(lv[2], pts, DevPoint{FA, FB, nullptr, G, (uint32_t)L, 0}, bntt::ITEM_GMUL);
-            add_points(lv[5], pts, DevPoint{Y, nullptr, s->tabs[L].scale, r.c, (uint32_t)L, 0}, bntt::ITEM_SCAT);
+            U128 *base = arena + r.arena_off * (kUnit / sizeof(U128));
+            U128 *FA = base, *FB = base + n;
+            U128 *TA = passes_for(L) > 1 ? base + 2 * n : nullptr, *TB = TA ? base + 3 * n : nullptr;
+            DevDesc da{}, db{}, di{};
+            da.x = r.a, da.y = FA, da.tmp = TA, da.twr = s->tabs[L].fwd;
+            db.x = r.b, db.y = FB, db.tmp = TB, db.twr = s->tabs[L].fwd;
+            // inverse: pass 0 gathers FA * FB, the last pass scatters into c
+            di.x = FA, di.x2 = FB, di.y = r.c, di.tmp = TA, di.twr = s->tabs[L].inv, di.scale = s->tabs[L].scale;
+            fwd.push_back(da);
+            fwd.push_back(db);
+            inv.push_back(di);
         }
         if (!fwd.empty()) {
-            add_transforms(lv, 0, descs, fwd, L);
-            add_transforms(lv, 3, descs, inv, L);
+            add_transforms(lv, 0, descs, fwd, L, 0);
+            add_transforms(lv, 2, descs, inv, L, bntt::IT_GATHER_MUL | bntt::IT_SCATTER_SCALE);
         }
     }
     size_t nitems = 0;
     for (auto &v : lv) nitems += v.size();
-    if (nitems > sl.cap_items || descs.size() > sl.cap_descs || pts.size() > sl.cap_pts)
+    if (nitems > sl.cap_items || descs.size() > sl.cap_descs)
         return bntt_set_error(BNTT_EINVAL, "internal: batch exceeds the slot capacity");
     WorkItem *it_h = reinterpret_cast<WorkItem *>(sl.stage_h);
     DevDesc *de_h = reinterpret_cast<DevDesc *>(sl.stage_h + sl.cap_items * sizeof(WorkItem));
-    DevPoint *pt_h =
-        reinterpret_cast<DevPoint *>(sl.stage_h + sl.cap_items * sizeof(WorkItem) + sl.cap_descs * sizeof(DevDesc));
-    std::vector<size_t> off(6);
+    std::vector<size_t> off(kLevels);
     size_t pos = 0;
-    for (int l = 0; l < 6; l++) {
+    for (int l = 0; l < kLevels; l++) {
         off[l] = pos;
         std::memcpy(it_h + pos, lv[l].data(), lv[l].size() * sizeof(WorkItem));
         pos += lv[l].size();
     }
     std::memcpy(de_h, descs.data(), descs.size() * sizeof(DevDesc));
-    std::memcpy(pt_h, pts.data(), pts.size() * sizeof(DevPoint));
     hipError_t e = hipMemcpyAsync(sl.items_d, it_h, nitems * sizeof(WorkItem), hipMemcpyHostToDevice, sl.stream);
     if (e == hipSuccess && !descs.empty())
         e = hipMemcpyAsync(sl.descs_d, de_h, descs.size() * sizeof(DevDesc), hipMemcpyHostToDevice, sl.stream);
-    if (e == hipSuccess && !pts.empty())
-        e = hipMemcpyAsync(sl.pts_d, pt_h, pts.size() * sizeof(DevPoint), hipMemcpyHostToDevice, sl.stream);
     if (e != hipSuccess) return bntt_set_hip_error(e, "hipMemcpyAsync of a batch");
-    for (int l = 0; l < 6; l++) {
+    for (int l = 0; l < kLevels; l++) {
         if (lv[l].empty()) continue;
         size_t lds = 0;
-        for (const WorkItem &w : lv[l])
-            if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, sizeof(U128) * bntt::lds_data_slots(w.k, w.Glog));
+        for (const WorkItem &w : lv[l]) lds = std::max(lds, sizeof(U128) * bntt::lds_data_slots(w.k, w.Glog));
         const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)s->grid_cap);
         hipLaunchKernelGGL(bntt::queue_kernel<true>, dim3(grid), dim3(bntt::kQueueThreads), lds, sl.stream,
-                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.pts_d, sl.ctl, s->md);
+                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.ctl, s->md);
         e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "serving kernel launch");
     }
@@ -443,7 +428,7 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
     const size_t cap_items = (size_t)s->max_batch * items_per_request(s->max_l) + 64;
     s->slots.resize(s->nslots + 1);
     for (auto &sl : s->slots)
-        if ((rc = make_slot(sl, cap_items, 3 * (size_t)s->max_batch, 2 * (size_t)s->max_batch))) {
+        if ((rc = make_slot(sl, cap_items, 3 * (size_t)s->max_batch))) {
             destroy_server(s);
             return rc;
         }

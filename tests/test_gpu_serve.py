@@ -17,8 +17,10 @@ UNIT = 256
 
 
 def req_bytes(L):
-    """Arena bytes one request of 2^L points holds while in flight."""
-    return -(-(4 << L) * 16 // UNIT) * UNIT
+    """Arena bytes one request of 2^L points holds while in flight: two
+    buffers of N residues up to 2^11 points (one-pass transforms), four above."""
+    bufs = 2 if L <= 11 else 4
+    return -(-(bufs << L) * 16 // UNIT) * UNIT
 
 
 def rand(n, q, seed):
