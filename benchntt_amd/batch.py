@@ -59,8 +59,17 @@ def _task_array(tasks):
     return arr
 
 
+TW_FLAGS = {"global": 0, "lds": 2, "wide": 4}
+
+
 class BatchPlan:
-    def __init__(self, tasks, muls=(), modulus: int = 0, mu: int = 0, tune=None, graph: bool = True):
+    """bntt_batch_plan_*: tasks are (x, y, tw) device tensors, muls (a, b, c);
+    tw_mode "global" (twiddles from a digit copy in device memory), "lds"
+    (staged in LDS where they keep three workgroups per CU) or "wide" (staged
+    up to 80 KiB of LDS)."""
+
+    def __init__(self, tasks, muls=(), modulus: int = 0, mu: int = 0, tune=None, graph: bool = True,
+                 tw_mode: str = "global"):
         self._keep = (list(tasks), list(muls))  # the buffers must outlive the plan
         tarr = _task_array(self._keep[0])
         marr = (MulTask * max(1, len(self._keep[1])))()
@@ -74,7 +83,7 @@ class BatchPlan:
         mm, mmu = U128.of(modulus), U128.of(mu)
         _lib.call("bntt_batch_plan_create", ctypes.byref(handle), tarr, len(self._keep[0]), marr,
                   len(self._keep[1]), ctypes.byref(mm), ctypes.byref(mmu), tune_arr, ntune,
-                  _lib.BNTT_PLAN_GRAPH if graph else 0)
+                  (_lib.BNTT_PLAN_GRAPH if graph else 0) | TW_FLAGS[tw_mode])
         self._h = handle
 
     def execute(self, stream=None) -> None:

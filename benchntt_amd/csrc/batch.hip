@@ -76,6 +76,7 @@ struct bntt_batch_plan {
     std::vector<size_t> level_items;     // items per level
     std::vector<size_t> level_offset;    // first item of each level in items_d
     std::vector<size_t> level_lds;       // dynamic LDS bytes of each level's launch
+    std::vector<bntt::TwArea> level_tw;  // staged-twiddle area of each level's launch
     std::vector<int> level_grid;         // workgroups of each level's launch
     WorkItem *items_d = nullptr;
     DevDesc *descs_d = nullptr;
@@ -169,7 +170,7 @@ int launch_levels(bntt_batch_plan *p, hipStream_t s)
         if (p->level_items[l] == 0) continue;
         hipLaunchKernelGGL(bntt::queue_kernel<false>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
                            p->level_lds[l], s, p->items_d + p->level_offset[l], (uint32_t)p->level_items[l],
-                           p->descs_d, p->muls_d, p->ctl_d, p->md);
+                           p->descs_d, p->muls_d, p->ctl_d, p->level_tw[l], p->md);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "queue kernel launch");
     }
@@ -431,9 +432,18 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             else
                 lv.push_back(nt[a_i++]);
         }
-        size_t lds = 0;
+        size_t data_slots = 0;
+        std::vector<size_t> needs;
         for (const WorkItem &w : lv)
-            if (w.kind == bntt::ITEM_NTT) lds = std::max(lds, item_lds_bytes(w.k, w.Glog));
+            if (w.kind == bntt::ITEM_NTT) {
+                data_slots = std::max(data_slots, bntt::lds_data_slots(w.k, w.Glog));
+                needs.push_back(bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, w.Glog)));
+            }
+        const size_t budget = (flags & BNTT_PLAN_TW_WIDE) ? bntt::kQueueMaxLds
+                              : (flags & BNTT_PLAN_TW_LDS) ? bntt::kQueueLds3
+                                                           : 0;
+        bntt::TwArea area;
+        size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), budget, &area);
         int per_cu = 0;
         hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<false>, bntt::kQueueThreads, lds);
         if (per_cu < 1) per_cu = 1;
@@ -441,6 +451,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         p->level_offset.push_back(all.size());
         p->level_items.push_back(lv.size());
         p->level_lds.push_back(lds);
+        p->level_tw.push_back(area);
         p->level_grid.push_back(grid);
         all.insert(all.end(), lv.begin(), lv.end());
     }
@@ -448,6 +459,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         p->level_items.pop_back();
         p->level_offset.pop_back();
         p->level_lds.pop_back();
+        p->level_tw.pop_back();
         p->level_grid.pop_back();
     }
     p->nitems = all.size();

@@ -12,7 +12,7 @@ namespace bntt {
 template <int ELOG, bool PM>
 __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict__ y, const U128 *__restrict__ x,
                                                                const U128 *__restrict__ tw, int L, int S, int k,
-                                                               int Glog, size_t total_groups, Mod md)
+                                                               int Glog, int klds, size_t total_groups, Mod md)
 {
     extern __shared__ U128 lds[];
     constexpr int E = 1 << ELOG;
@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     const uint32_t W = 1u << wlog;
     const uint32_t P0start = (wlog == Glog) ? (mbase & ((1u << S) - 1u)) : 0u;
     R31 *tws = reinterpret_cast<R31 *>(lds + lds_data_slots(k, Glog));
-    big |= stage_twiddles(tws, tw, S, k, W, P0start);
+    big |= stage_twiddles(tws, tw, S, k, wlog, P0start, klds);
     const bool slow = __syncthreads_or(big) || !PM;
 
     TwView tv;
@@ -63,7 +63,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     tv.glob = tw;
     tv.W = W;
     if (!slow) {
-        if constexpr (PM) run_group_network<ELOG, TW_LDS, PM>(lds, tv, k, S, mbase, md);
+        if constexpr (PM) run_group_network<ELOG, TW_LDS, PM, true>(lds, tv, k, S, mbase, md, klds);
     } else {
         run_group_network<ELOG, TW_SLOW, PM>(lds, tv, k, S, mbase, md);
     }
@@ -84,7 +84,12 @@ static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int
     const size_t total_groups = count << (L - k);
     const size_t blocks = (total_groups + ((size_t)1 << Glog) - 1) >> Glog;
     const int threads = (1 << (Glog + k)) >> ELOG;
-    const size_t lds_bytes = pass_lds_bytes(k, S, Glog);
+    // stage the tables of as many stages as keep three 256-thread workgroups
+    // on a CU; the remaining stages read the caller's table
+    int klds = k;
+    if (threads == 256)
+        while (klds > 1 && pass_lds_bytes(k, S, Glog, klds) > kLds3Bytes) klds--;
+    const size_t lds_bytes = pass_lds_bytes(k, S, Glog, klds);
     if (threads < 1 || threads > kMaxThreads || blocks > 0x7fffffffu || lds_bytes > kMaxLdsBytes)
         return hipErrorInvalidConfiguration;
     // raise the dynamic LDS limit once per instantiation (thread-safe static init)
@@ -93,7 +98,7 @@ static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int
                                                        (int)kMaxLdsBytes);
     if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL((ntt_pass_kernel<ELOG, PM>), dim3((unsigned)blocks), dim3(threads), lds_bytes, stream, y, x,
-                       tw, L, S, k, Glog, total_groups, md);
+                       tw, L, S, k, Glog, klds, total_groups, md);
     return hipGetLastError();
 }
 

@@ -50,6 +50,9 @@ constexpr int kMaxThreads = 512;
 // Dynamic LDS of the largest launch: a 4096-point transform and its 4095
 // staged twiddles, padded (the CU has 160 KiB).
 constexpr size_t kMaxLdsBytes = 148 * 1024;
+// dynamic LDS per workgroup that leaves room for three on a CU (160 KiB / 3,
+// less the kernel's static LDS)
+constexpr size_t kLds3Bytes = 53 * 1024;
 
 // LDS placement.  Data element i of groups of 2^k elements sits at
 //   lidx(i, k) = i + (i >> 3) + (i >> k):
@@ -84,7 +87,13 @@ __device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFF
 //   TW_MONT:   like TW_GLOBAL, but the digit table holds tw * 2^124 mod q and
 //              the arithmetic is the exact Montgomery product with canonical
 //              add/sub (polynomial-multiply path, any odd q < 2^122).
-enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2, TW_MONT = 3 };
+//   TW_MONT_LDS: TW_MONT arithmetic with TW_LDS staged tables.
+//   TW_G128:   the caller's table, entries converted to digits in registers,
+//              fast arithmetic (stages left unstaged by a TW_LDS workgroup).
+enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2, TW_MONT = 3, TW_MONT_LDS = 4, TW_G128 = 5 };
+
+__host__ __device__ constexpr bool tw_in_lds(int mode) { return mode == TW_LDS || mode == TW_MONT_LDS; }
+__host__ __device__ constexpr bool tw_mont(int mode) { return mode == TW_MONT || mode == TW_MONT_LDS; }
 
 struct TwView {
     const R31 *lds;     // TW_LDS tables
@@ -103,10 +112,14 @@ __device__ __forceinline__ void fetch_twiddles(R31 (&t)[(1 << KP) - 1], const Tw
 #pragma unroll
     for (int rp = 0; rp < KP; rp++) {
         const int r = r0 + rp;
-        if constexpr (MODE == TW_LDS) {
+        if constexpr (tw_in_lds(MODE)) {
             const int tb = (int)(tv.W * ((1u << r) - 1u) + (rel << r) + (Mlow << rp));
 #pragma unroll
             for (int j = 0; j < (1 << rp); j++) t[(1 << rp) - 1 + j] = tv.lds[tidx(tb + j)];
+        } else if constexpr (MODE == TW_G128) {
+            const U128 *tb = tv.glob + (1u << (S + r)) + (P << rp);
+#pragma unroll
+            for (int j = 0; j < (1 << rp); j++) t[(1 << rp) - 1 + j] = to_r31(ld16(tb + j));
         } else {
             const R31 *tb = tv.gr31 + (1u << (S + r)) + (P << rp);
 #pragma unroll
@@ -225,7 +238,7 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
         const int sg = threadIdx.x + q * T;
         const int g = sg >> sub_log;
         const int Mp = sg & ((1 << sub_log) - 1);
-        if constexpr (MODE == TW_MONT) {
+        if constexpr (tw_mont(MODE)) {
             reg_network_mont<KP>(v[q], t[q], md);
         } else if constexpr (MODE != TW_SLOW) {
             reg_network_fast<KP>(v[q], t[q], md);
@@ -248,51 +261,98 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
     __syncthreads();
 }
 
-template <int ELOG, int MODE, bool PM>
+// A TW_LDS network whose tables are staged only for its first klds stages
+// (PARTIAL) runs the sub-passes that reach beyond them with TW_G128.
+template <int ELOG, int MODE, bool PM, bool PARTIAL>
+__device__ __forceinline__ void run_subpass(U128 *lds, const TwView &tv, int k, int S, int r0, int kp, bool lds_tw,
+                                            uint32_t mbase, const Mod &md)
+{
+    if constexpr (MODE == TW_LDS && PARTIAL) {
+        if (!lds_tw) {
+            if (kp == ELOG) {
+                lds_subpass<ELOG, ELOG, TW_G128, PM>(lds, tv, k, S, r0, mbase, md);
+            } else if constexpr (ELOG >= 2) {
+                if (kp == 1) {
+                    lds_subpass<ELOG, 1, TW_G128, PM>(lds, tv, k, S, r0, mbase, md);
+                } else if constexpr (ELOG >= 3) {
+                    if (kp == 2) lds_subpass<ELOG, 2, TW_G128, PM>(lds, tv, k, S, r0, mbase, md);
+                }
+            }
+            return;
+        }
+    }
+    if (kp == ELOG) {
+        lds_subpass<ELOG, ELOG, MODE, PM>(lds, tv, k, S, r0, mbase, md);
+    } else if constexpr (ELOG >= 2) {
+        if (kp == 1) {
+            lds_subpass<ELOG, 1, MODE, PM>(lds, tv, k, S, r0, mbase, md);
+        } else if constexpr (ELOG >= 3) {
+            if (kp == 2) lds_subpass<ELOG, 2, MODE, PM>(lds, tv, k, S, r0, mbase, md);
+        }
+    }
+}
+
+template <int ELOG, int MODE, bool PM, bool PARTIAL = false>
 __device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase,
-                                                  const Mod &md)
+                                                  const Mod &md, int klds = 32)
 {
     int r0 = 0;
     while (r0 < k) {
         const int kp = (k - r0) < ELOG ? (k - r0) : ELOG;
-        if (kp == ELOG) {
-            lds_subpass<ELOG, ELOG, MODE, PM>(lds, tv, k, S, r0, mbase, md);
-        } else if constexpr (ELOG >= 2) {
-            if (kp == 1) {
-                lds_subpass<ELOG, 1, MODE, PM>(lds, tv, k, S, r0, mbase, md);
-            } else if constexpr (ELOG >= 3) {
-                if (kp == 2) lds_subpass<ELOG, 2, MODE, PM>(lds, tv, k, S, r0, mbase, md);
-            }
-        }
+        run_subpass<ELOG, MODE, PM, PARTIAL>(lds, tv, k, S, r0, kp, r0 + kp <= klds, mbase, md);
         r0 += kp;
     }
 }
 
-// Stage, as digits, the twiddles that stages [S, S+k) use for W consecutive
-// group prefixes starting at P0start: stage table r (2^r entries per prefix)
-// sits at LDS offset W*(2^r - 1).  Returns whether any entry is not small.
-__device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict__ tw, int S, int k, uint32_t W,
-                                               uint32_t P0start)
+// Stage, as digits, the twiddles that stages [S, S+klds) use for W = 2^wlog
+// consecutive group prefixes starting at P0start: stage table r (2^r entries
+// per prefix) sits at LDS offset W*(2^r - 1).  Returns whether any entry of
+// stages [S, S+k) is not small.
+__device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict__ tw, int S, int k, int wlog,
+                                               uint32_t P0start, int klds)
 {
     bool big = false;
-    const int ntw = (int)(W * ((1u << k) - 1u));
+    const int ntw = (int)(((1u << k) - 1u) << wlog);
+    const int nstaged = (int)(((1u << (klds < k ? klds : k)) - 1u) << wlog);  // stages r < klds come first
     for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
         // e = W*(2^r - 1) + rel*2^r + j  ->  stage r, offset within the stage table
-        const uint32_t u = (uint32_t)e / W + 1u;  // in [2^r, 2^(r+1))
+        const uint32_t u = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
         const int r = 31 - __builtin_clz(u);
-        const uint32_t off = (uint32_t)e - W * ((1u << r) - 1u);
+        const uint32_t off = (uint32_t)e - (((1u << r) - 1u) << wlog);
         const U128 a = ld16(tw + (1u << (S + r)) + (P0start << r) + off);
         big |= is_big(a);
-        tws[tidx(e)] = to_r31(a);
+        if (e < nstaged) tws[tidx(e)] = to_r31(a);
     }
     return big;
 }
 
-// LDS bytes of one workgroup: the data, then the staged digit tables.
-__host__ __device__ inline size_t pass_lds_bytes(int k, int S, int Glog)
+// The same staging from a table already in digits (no check: the caller
+// knows whether the table is small).
+__device__ __forceinline__ void stage_twiddles_r31(R31 *tws, const R31 *__restrict__ twr, int S, int k, int wlog,
+                                                   uint32_t P0start)
+{
+    const int ntw = (int)(((1u << k) - 1u) << wlog);
+    for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
+        const uint32_t u = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
+        const int r = 31 - __builtin_clz(u);
+        const uint32_t off = (uint32_t)e - (((1u << r) - 1u) << wlog);
+        tws[tidx(e)] = twr[(1u << (S + r)) + (P0start << r) + off];
+    }
+}
+
+// Digit slots of the staged tables of k stages for 2^wlog prefixes.
+__host__ __device__ inline size_t tw_slots(int k, int wlog)
+{
+    const size_t ntw = (((size_t)1 << k) - 1) << wlog;
+    return ntw + ntw / 8 + 1;
+}
+
+// LDS bytes of one workgroup: the data, then the digit tables of the first
+// klds stages.
+__host__ __device__ inline size_t pass_lds_bytes(int k, int S, int Glog, int klds)
 {
     const int wlog = Glog < S ? Glog : S;
-    const size_t ntw = ((size_t)1 << wlog) * (((size_t)1 << k) - 1);
+    const size_t ntw = ((size_t)1 << wlog) * (((size_t)1 << (klds < k ? klds : k)) - 1);
     return sizeof(U128) * lds_data_slots(k, Glog) + sizeof(R31) * (ntw + ntw / 8 + 1);
 }
 

@@ -23,8 +23,11 @@ namespace bntt {
 constexpr int kQueueThreads = 256;
 constexpr int kQueueMaxElog = 3;
 constexpr int kQueueTile = kQueueThreads << kQueueMaxElog;  // 2048 values
-// padded data tile (the twiddles are read from device memory)
-constexpr size_t kQueueMaxLds = 64 * 1024;
+// padded data tile, then (when a launch reserves room) staged digit twiddles
+constexpr size_t kQueueMaxLds = 80 * 1024;
+// dynamic LDS that keeps three workgroups on a CU (160 KiB / 3, less the
+// static arrays of the kernel, up to 6.2 KiB)
+constexpr size_t kQueueLds3 = 46 * 1024;
 
 enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
 enum ItemFlags : uint8_t { IT_GATHER_MUL = 1, IT_SCATTER_SCALE = 2 };
@@ -80,16 +83,49 @@ constexpr int kTwChunk = 4096;
 
 template <int ELOG, bool MONT>
 __device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, bool slow,
-                                                 const Mod &md)
+                                                 bool staged, const Mod &md)
 {
     if constexpr (MONT) {
-        run_group_network<ELOG, TW_MONT, true>(lds, tv, k, S, mbase, md);
-    } else {
-        if (!slow)
-            run_group_network<ELOG, TW_GLOBAL, true>(lds, tv, k, S, mbase, md);
+        if (staged)
+            run_group_network<ELOG, TW_MONT_LDS, true>(lds, tv, k, S, mbase, md);
         else
+            run_group_network<ELOG, TW_MONT, true>(lds, tv, k, S, mbase, md);
+    } else {
+        if (slow)
             run_group_network<ELOG, TW_SLOW, true>(lds, tv, k, S, mbase, md);
+        else if (staged)
+            run_group_network<ELOG, TW_LDS, true>(lds, tv, k, S, mbase, md);
+        else
+            run_group_network<ELOG, TW_GLOBAL, true>(lds, tv, k, S, mbase, md);
     }
+}
+
+// LDS layout of a launch: the data tile in U128 slots [0, tw_off), then room
+// for tw_cap staged digit twiddles.  The staged tables stay valid across the
+// items of a workgroup until an item needs different ones (key).
+struct TwArea {
+    uint32_t tw_off;
+    uint32_t tw_cap;
+};
+
+// Twiddle-table prefixes staged for a transform item (see do_ntt_item).
+__host__ __device__ inline int item_tw_wlog(int L, int k, int S, int Glog)
+{
+    return L == k ? 0 : (Glog < S ? Glog : S);
+}
+
+// Layout of a launch over items whose data tiles need at most data_slots and
+// whose staged tables need tw_needs[i] digit slots: the twiddle area holds the
+// largest need that fits `budget` bytes of dynamic LDS (items needing more read
+// their twiddles from device memory).  Returns the dynamic LDS bytes.
+inline size_t plan_tw_area(size_t data_slots, const size_t *tw_needs, size_t n, size_t budget, TwArea *area)
+{
+    size_t cap = 0;
+    for (size_t i = 0; i < n; i++)
+        if (tw_needs[i] > cap && sizeof(U128) * data_slots + sizeof(R31) * tw_needs[i] <= budget) cap = tw_needs[i];
+    area->tw_off = (uint32_t)data_slots;
+    area->tw_cap = (uint32_t)cap;
+    return sizeof(U128) * data_slots + sizeof(R31) * cap;
 }
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
@@ -99,8 +135,8 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
 
 // One transform item: load, network, store.
 template <bool MONT>
-__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, const WorkItem &it,
-                            const DevDesc *__restrict__ descs, const Mod &md)
+__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, uint64_t *s_twkey,
+                            const WorkItem &it, const DevDesc *__restrict__ descs, TwArea area, const Mod &md)
 {
     const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
     const int G = 1 << Glog, E = 1 << elog;
@@ -179,15 +215,35 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     tv.gr31 = d0.twr;
     tv.glob = d0.tw;
     tv.W = 1;
+    // the W = 2^wlog group prefixes of this item start at P0start (groups of
+    // a multi-pass item are consecutive M, the first a multiple of G)
+    const int wlog = (mlog == 0) ? 0 : (Glog < S ? Glog : S);
+    const uint32_t P0start = (mlog == 0 || S <= Glog) ? 0u : (uint32_t)(it.first & ((1ull << S) - 1ull));
+    const bool staged = !slow && area.tw_cap >= tw_slots(k, wlog);
+    if (staged) {
+        R31 *tws = reinterpret_cast<R31 *>(lds + area.tw_off);
+        const uint64_t key = (uint64_t)S | ((uint64_t)k << 8) | ((uint64_t)wlog << 16) | ((uint64_t)P0start << 32);
+        if (s_twkey[0] != reinterpret_cast<uint64_t>(d0.twr) || s_twkey[1] != key) {
+            __syncthreads();  // every thread has read the key
+            stage_twiddles_r31(tws, d0.twr, S, k, wlog, P0start);
+            if (threadIdx.x == 0) {
+                s_twkey[0] = reinterpret_cast<uint64_t>(d0.twr);
+                s_twkey[1] = key;
+            }
+            __syncthreads();
+        }
+        tv.lds = tws;
+        tv.W = 1u << wlog;
+    }
     switch (elog) {
     case 1:
-        run_network_elog<1, MONT>(lds, tv, k, S, mbase, slow, md);
+        run_network_elog<1, MONT>(lds, tv, k, S, mbase, slow, staged, md);
         break;
     case 2:
-        run_network_elog<2, MONT>(lds, tv, k, S, mbase, slow, md);
+        run_network_elog<2, MONT>(lds, tv, k, S, mbase, slow, staged, md);
         break;
     default:
-        run_network_elog<3, MONT>(lds, tv, k, S, mbase, slow, md);
+        run_network_elog<3, MONT>(lds, tv, k, S, mbase, slow, staged, md);
         break;
     }
     // LDS -> HBM
@@ -232,13 +288,15 @@ template <bool MONT>
 __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
                                                                  const DevDesc *__restrict__ descs,
                                                                  const DevMul *__restrict__ muls, QueueCtl *ctl,
-                                                                 Mod md)
+                                                                 TwArea area, Mod md)
 {
     extern __shared__ U128 lds[];
     __shared__ uint32_t s_ticket;
+    __shared__ uint64_t s_twkey[2];  // tables staged in the twiddle area (none yet)
     __shared__ const U128 *s_x[kQueueThreads];  // buffers of the transforms of a packed item
     __shared__ U128 *s_y[kQueueThreads];
     __shared__ const U128 *s_x2[MONT ? kQueueThreads : 1];
+    if (threadIdx.x == 0) s_twkey[0] = s_twkey[1] = ~0ull;
     for (;;) {
         if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->ticket, 1u);
         __syncthreads();
@@ -247,7 +305,7 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
         if (t >= nitems) break;
         const WorkItem it = items[t];
         if (it.kind == ITEM_NTT) {
-            do_ntt_item<MONT>(lds, s_x, s_y, s_x2, it, descs, md);
+            do_ntt_item<MONT>(lds, s_x, s_y, s_x2, s_twkey, it, descs, area, md);
         } else if constexpr (!MONT) {
             do_mul_item(it, muls, md);
         }

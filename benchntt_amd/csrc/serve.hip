@@ -158,7 +158,9 @@ struct bntt_pm_server {
     char *arena_d = nullptr;
     Arena arena;
     std::vector<Slot> slots;  // nslots for traces + 1 for bntt_pm_multiply
-    int grid_cap = 0;
+    int grid_cap = 0;         // resident workgroups with up to kQueueLds3 of dynamic LDS
+    int grid_cap2 = 0;        // ... with up to kQueueMaxLds
+    size_t tw_budget = 0;     // twiddles from device memory (staging measured slower)
 };
 
 namespace {
@@ -359,11 +361,17 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
     if (e != hipSuccess) return bntt_set_hip_error(e, "hipMemcpyAsync of a batch");
     for (int l = 0; l < kLevels; l++) {
         if (lv[l].empty()) continue;
-        size_t lds = 0;
-        for (const WorkItem &w : lv[l]) lds = std::max(lds, sizeof(U128) * bntt::lds_data_slots(w.k, w.Glog));
-        const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)s->grid_cap);
+        size_t data_slots = 0;
+        std::vector<size_t> needs;
+        for (const WorkItem &w : lv[l]) {
+            data_slots = std::max(data_slots, bntt::lds_data_slots(w.k, w.Glog));
+            needs.push_back(bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, w.Glog)));
+        }
+        bntt::TwArea area;
+        const size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), s->tw_budget, &area);
+        const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)(lds > bntt::kQueueLds3 ? s->grid_cap2 : s->grid_cap));
         hipLaunchKernelGGL(bntt::queue_kernel<true>, dim3(grid), dim3(bntt::kQueueThreads), lds, sl.stream,
-                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.ctl, s->md);
+                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.ctl, area, s->md);
         e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "serving kernel launch");
     }
@@ -434,10 +442,18 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
         }
     hipDeviceProp_t prop;
     hipGetDeviceProperties(&prop, s->dev);
-    int per_cu = 0;
+    int per_cu = 0, per_cu2 = 0;
     hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<true>, bntt::kQueueThreads,
-                                                 sizeof(U128) * bntt::lds_data_slots(8, 3));
+                                                 bntt::kQueueLds3);
+    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bntt::queue_kernel<true>, bntt::kQueueThreads,
+                                                 bntt::kQueueMaxLds);
     s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);
+    s->grid_cap2 = prop.multiProcessorCount * std::max(per_cu2, 1);
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
+        destroy_server(s);
+        return bntt_set_error(BNTT_EHIP, "hipFuncSetAttribute");
+    }
     if (hipDeviceSynchronize() != hipSuccess) {
         destroy_server(s);
         return bntt_set_error(BNTT_EHIP, "server set-up failed");

@@ -138,6 +138,13 @@ typedef struct bntt_batch_plan bntt_batch_plan;
 
 #define BNTT_PLAN_GRAPH 1 /* capture the launches in a hipGraph on the first
                              execute with a non-null stream, replay after */
+/* Where transform items read their twiddles.  Default: a digit copy in device
+ * memory (L1/L2-resident; measured fastest on MI355X).  TW_LDS: staged in LDS
+ * when the tables fit beside the data tile with three workgroups per CU.
+ * TW_WIDE: staged whenever they fit in 80 KiB of LDS with the data (two
+ * workgroups per CU). */
+#define BNTT_PLAN_TW_LDS 2
+#define BNTT_PLAN_TW_WIDE 4
 
 /* Build a plan: device work lists, intermediates for multi-pass sizes, and
  * per-size kernel choices (tune may be NULL for built-in defaults).  The plan

@@ -53,12 +53,16 @@ def check_all(orc, wl, y, c, threads=16, every=1):
     assert not errs, errs[:5]
 
 
-def test_small_uneven_batch(gpu, orc):
+@pytest.mark.parametrize("tw_mode", ["global", "lds", "wide"])
+def test_small_uneven_batch(gpu, orc, tw_mode):
+    """Sizes 2^1..2^16 with their products, for each placement of the
+    twiddles (staged in LDS within the three- or two-workgroup budget, or read
+    from device memory)."""
     from benchntt_amd.batch import BatchPlan
 
     wl = uneven_pairs(80, QN, seed=11, lo=1, hi=16)
     dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
-    plan = BatchPlan(tasks, muls, QN, MUN, graph=False)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=False, tw_mode=tw_mode)
     plan.execute()
     gpu.cuda.synchronize()
     check_all(orc, wl, from_dev(dy), from_dev(dc))
@@ -93,13 +97,14 @@ def test_graph_replays_follow_inputs(gpu, orc):
     {l: (2, 1) for l in range(2, 11)} | {l: (3, 3) for l in range(11, 17)},
     {l: (3, 2) for l in range(6, 17)},
 ])
-def test_tuning_choices_agree(gpu, orc, tune):
+@pytest.mark.parametrize("tw_mode", ["global", "wide"])
+def test_tuning_choices_agree(gpu, orc, tune, tw_mode):
     """Every radix / pass-count choice gives the reference's outputs."""
     from benchntt_amd.batch import BatchPlan
 
     wl = uneven_pairs(60, QN, seed=7, lo=1, hi=16)
     dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
-    plan = BatchPlan(tasks, muls, QN, MUN, tune=tune, graph=False)
+    plan = BatchPlan(tasks, muls, QN, MUN, tune=tune, graph=False, tw_mode=tw_mode)
     plan.execute()
     gpu.cuda.synchronize()
     check_all(orc, wl, from_dev(dy), from_dev(dc))
@@ -116,7 +121,7 @@ def test_large_sizes_three_passes(gpu, orc):
         dy = gpu.empty_like(dx)
         tasks.append((dx, dy, dtw))
         xs.append(x), tws.append(tw), ys.append(dy)
-    plan = BatchPlan(tasks, [], QN, MUN, graph=False)
+    plan = BatchPlan(tasks, [], QN, MUN, graph=False, tw_mode="wide")
     plan.execute()
     gpu.cuda.synchronize()
     for x, tw, dy in zip(xs, tws, ys):
@@ -133,11 +138,12 @@ def test_edge_values_in_batch(gpu, orc):
     wl.x[5::97, 1] = np.uint64(0xFFFFFFFFFFFFFFFF)
     wl.x[7::131] = [QN & (2**64 - 1), QN >> 64]
     dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
-    plan = BatchPlan(tasks, muls, QN, MUN, graph=False)
-    plan.execute()
-    gpu.cuda.synchronize()
-    check_all(orc, wl, from_dev(dy), from_dev(dc))
-    plan.close()
+    for mode in ("global", "lds", "wide"):
+        plan = BatchPlan(tasks, muls, QN, MUN, graph=False, tw_mode=mode)
+        plan.execute()
+        gpu.cuda.synchronize()
+        check_all(orc, wl, from_dev(dy), from_dev(dc))
+        plan.close()
 
 
 def test_launch_each_matches(gpu, orc):
