@@ -2,10 +2,10 @@
 // the uneven-batch planner (batch.hip, reference arithmetic) and the
 // polynomial-multiply server (serve.hip, exact Montgomery arithmetic).
 //
-// A launch processes one list of equal-size work items: a grid sized to the
-// resident capacity of the chip whose workgroups take items from an atomic
-// ticket until the list is exhausted; the last workgroup to leave rearms the
-// counters for the next launch.  Kinds of item:
+// A launch processes one list of work items: a grid sized to the resident
+// capacity of the chip (at most one workgroup per item) whose workgroups take
+// items from an atomic counter until the list is exhausted; the last draw
+// rearms the counter for the next launch.  Kinds of item:
 //   ITEM_NTT   stages [S, S+k) of the forward network for G groups (several
 //              whole transforms when k = log2 N, else blocks of one transform)
 //   ITEM_MUL   2048 elements of c = a * b mod q
@@ -62,9 +62,8 @@ struct WorkItem {
 static_assert(sizeof(WorkItem) == 32, "work item layout");
 
 struct QueueCtl {
-    uint32_t ticket;
-    uint32_t done;
-    uint32_t pad[2];
+    uint32_t ticket;  // dynamic draws of the running launch; 0 between launches
+    uint32_t pad[3];
 };
 
 // Conversion of twiddle tables to digits.
@@ -297,28 +296,28 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
     __shared__ U128 *s_y[kQueueThreads];
     __shared__ const U128 *s_x2[MONT ? kQueueThreads : 1];
     if (threadIdx.x == 0) s_twkey[0] = s_twkey[1] = ~0ull;
-    for (;;) {
-        if (threadIdx.x == 0) s_ticket = atomicAdd(&ctl->ticket, 1u);
-        __syncthreads();
-        const uint32_t t = s_ticket;
-        __syncthreads();
-        if (t >= nitems) break;
+    // Workgroup b starts with item b; after each item it draws from the
+    // counter, draw c giving item gridDim + c.  A device-wide atomic is
+    // performed beyond the XCDs' L2 caches, so the first round draws nothing.
+    // Every processed item is followed by exactly one draw, so the draws are
+    // 0 .. nitems-1: the workgroup drawing nitems-1 is the last to touch the
+    // counter and rearms it for the next launch on this counter block.
+    uint32_t t = blockIdx.x;
+    while (t < nitems) {
         const WorkItem it = items[t];
         if (it.kind == ITEM_NTT) {
             do_ntt_item<MONT>(lds, s_x, s_y, s_x2, s_twkey, it, descs, area, md);
         } else if constexpr (!MONT) {
             do_mul_item(it, muls, md);
         }
-    }
-    // every workgroup has drawn a ticket past the end; the last one to leave
-    // rearms the counters for the next launch on this counter block
-    if (threadIdx.x == 0) {
-        __threadfence();
-        const uint32_t d = atomicAdd(&ctl->done, 1u);
-        if (d == gridDim.x - 1) {
-            atomicExch(&ctl->ticket, 0u);
-            atomicExch(&ctl->done, 0u);
+        if (threadIdx.x == 0) {
+            const uint32_t c = atomicAdd(&ctl->ticket, 1u);
+            if (c == nitems - 1) atomicExch(&ctl->ticket, 0u);
+            s_ticket = gridDim.x + c;
         }
+        __syncthreads();
+        t = s_ticket;
+        __syncthreads();
     }
 }
 
