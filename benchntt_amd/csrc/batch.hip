@@ -155,8 +155,6 @@ void free_plan(bntt_batch_plan *p)
     delete p;
 }
 
-size_t item_lds_bytes(int k, int Glog) { return sizeof(U128) * bntt::lds_data_slots(k, Glog); }
-
 int launch_levels(bntt_batch_plan *p, hipStream_t s)
 {
     if (p->ntwjobs) {
@@ -204,7 +202,10 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     }
 
     bntt_batch_plan *p = new bntt_batch_plan();
-    hipGetDevice(&p->dev);
+    if (hipGetDevice(&p->dev) != hipSuccess) {
+        delete p;
+        return bntt_set_error(BNTT_EHIP, "hipGetDevice");
+    }
     p->md = bntt_make_mod(m, mu);
     p->use_graph = (flags & BNTT_PLAN_GRAPH) != 0;
 
@@ -411,8 +412,19 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     int grid_cap = 0;
     {
         hipDeviceProp_t prop;
-        hipGetDeviceProperties(&prop, p->dev);
+        if (hipGetDeviceProperties(&prop, p->dev) != hipSuccess) {
+            free_plan(p);
+            return bntt_set_error(BNTT_EHIP, "hipGetDeviceProperties");
+        }
         grid_cap = prop.multiProcessorCount;
+    }
+    // dynamic LDS above 64 KiB must be allowed before occupancy is queried
+    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<false>),
+                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                       (int)bntt::kQueueMaxLds);
+    if (attr != hipSuccess) {
+        free_plan(p);
+        return bntt_set_hip_error(attr, "hipFuncSetAttribute");
     }
     const size_t nlevels = 5;
     for (size_t l = 0; l < nlevels; l++) {
@@ -445,8 +457,9 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         bntt::TwArea area;
         size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), budget, &area);
         int per_cu = 0;
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<false>, bntt::kQueueThreads, lds);
-        if (per_cu < 1) per_cu = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<false>, bntt::kQueueThreads,
+                                                         lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
         const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), (size_t)grid_cap * per_cu);
         p->level_offset.push_back(all.size());
         p->level_items.push_back(lv.size());
@@ -469,13 +482,6 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             free_plan(p);
             return bntt_set_error(BNTT_EINVAL, "internal: batch item exceeds the LDS budget");
         }
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<false>),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)bntt::kQueueMaxLds);
-    if (attr != hipSuccess) {
-        free_plan(p);
-        return bntt_set_hip_error(attr, "hipFuncSetAttribute");
-    }
     if ((rc = upload(&p->items_d, all)) || (rc = upload(&p->descs_d, packed)) || (rc = upload(&p->muls_d, mv))) {
         free_plan(p);
         return rc;

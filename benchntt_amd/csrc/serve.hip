@@ -407,7 +407,10 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
         cfg->max_inflight < 1 || cfg->max_inflight > 8)
         return bntt_set_error(BNTT_EINVAL, "bad server configuration");
     bntt_pm_server *s = new bntt_pm_server();
-    hipGetDevice(&s->dev);
+    if (hipGetDevice(&s->dev) != hipSuccess) {
+        delete s;
+        return bntt_set_error(BNTT_EHIP, "hipGetDevice");
+    }
     if (!s->hm.init(((u128)q->hi << 64) | q->lo)) {
         delete s;
         return bntt_set_error(BNTT_EINVAL, "q must be odd, at least 3 and below 2^122");
@@ -441,19 +444,21 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
             return rc;
         }
     hipDeviceProp_t prop;
-    hipGetDeviceProperties(&prop, s->dev);
-    int per_cu = 0, per_cu2 = 0;
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<true>, bntt::kQueueThreads,
-                                                 bntt::kQueueLds3);
-    hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bntt::queue_kernel<true>, bntt::kQueueThreads,
-                                                 bntt::kQueueMaxLds);
-    s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);
-    s->grid_cap2 = prop.multiProcessorCount * std::max(per_cu2, 1);
-    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<true>),
+    if (hipGetDeviceProperties(&prop, s->dev) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
         destroy_server(s);
-        return bntt_set_error(BNTT_EHIP, "hipFuncSetAttribute");
+        return bntt_set_error(BNTT_EHIP, "querying the device for the serving kernel failed");
     }
+    int per_cu = 0, per_cu2 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<true>, bntt::kQueueThreads,
+                                                     bntt::kQueueLds3) != hipSuccess)
+        per_cu = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bntt::queue_kernel<true>, bntt::kQueueThreads,
+                                                     bntt::kQueueMaxLds) != hipSuccess)
+        per_cu2 = 1;
+    s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);
+    s->grid_cap2 = prop.multiProcessorCount * std::max(per_cu2, 1);
     if (hipDeviceSynchronize() != hipSuccess) {
         destroy_server(s);
         return bntt_set_error(BNTT_EHIP, "server set-up failed");
