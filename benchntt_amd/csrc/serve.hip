@@ -111,7 +111,15 @@ class Arena {
 constexpr size_t kUnit = 256;
 
 int passes_for(int L) { return L <= 11 ? 1 : (L + 7) / 8; }
-int elog_for(int L) { return L < 3 ? L : 3; }
+// Points per thread per register sub-pass, 2^elog: `want` (3 normally, 2 for
+// small batches: twice the items, each half as long), within what a size
+// allows (a one-pass transform must fit the 256 << elog values of an item).
+int elog_for(int L, int want)
+{
+    int e = want < 3 ? want : 3;
+    if (passes_for(L) == 1 && e < L - 8) e = L - 8;
+    return e < L ? e : L;
+}
 
 size_t request_units(int L)
 {
@@ -120,14 +128,14 @@ size_t request_units(int L)
 }
 
 // Upper bound of work items one request contributes (three transforms).
-size_t items_per_request(int L)
+size_t items_per_request(int L, int want)
 {
     const int np = passes_for(L);
     if (np == 1) return 3;
     size_t per_transform = 0;
     for (int p = 0, S = 0; p < np; p++) {
         const int k = (L - S) / (np - p);
-        const int Glog = 8 + elog_for(L) - k;
+        const int Glog = 8 + elog_for(L, want) - k;
         per_transform += (((size_t)1 << (L - k)) + ((size_t)1 << Glog) - 1) >> Glog;
         S += k;
     }
@@ -263,9 +271,9 @@ struct ReqView {
 
 // Transforms of one size sharing one table: packed several per item when single-pass.
 void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector<DevDesc> &descs,
-                    const std::vector<DevDesc> &group, int L, uint8_t flags)
+                    const std::vector<DevDesc> &group, int L, uint8_t flags, int want)
 {
-    const int np = passes_for(L), elog = elog_for(L);
+    const int np = passes_for(L), elog = elog_for(L, want);
     if (np == 1) {
         const int Glog = 8 + elog - L;
         for (size_t i = 0; i < group.size(); i += ((size_t)1 << Glog)) {
@@ -319,6 +327,11 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
     std::vector<std::vector<WorkItem>> lv(kLevels);
     std::vector<DevDesc> descs;
     U128 *arena = reinterpret_cast<U128 *>(s->arena_d);
+    // a batch whose radix-8 items would leave resident workgroups idle runs
+    // 1024-value items: each level finishes in about half the time
+    size_t items8 = 0;
+    for (const ReqView &r : reqs) items8 += items_per_request(r.L, 3);
+    const int want = items8 < 3 * (size_t)s->grid_cap ? 2 : 3;
     for (int L = s->min_l; L <= s->max_l; L++) {
         std::vector<DevDesc> fwd, inv;
         for (const ReqView &r : reqs) {
@@ -337,8 +350,8 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
             inv.push_back(di);
         }
         if (!fwd.empty()) {
-            add_transforms(lv, 0, descs, fwd, L, 0);
-            add_transforms(lv, 2, descs, inv, L, bntt::IT_GATHER_MUL | bntt::IT_SCATTER_SCALE);
+            add_transforms(lv, 0, descs, fwd, L, 0, want);
+            add_transforms(lv, 2, descs, inv, L, bntt::IT_GATHER_MUL | bntt::IT_SCATTER_SCALE, want);
         }
     }
     size_t nitems = 0;
@@ -436,7 +449,7 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
         return bntt_set_error(BNTT_ENOMEM, "hipMalloc of the serving arena failed");
     }
     s->arena.reset(units);
-    const size_t cap_items = (size_t)s->max_batch * items_per_request(s->max_l) + 64;
+    const size_t cap_items = (size_t)s->max_batch * items_per_request(s->max_l, 2) + 64;
     s->slots.resize(s->nslots + 1);
     for (auto &sl : s->slots)
         if ((rc = make_slot(sl, cap_items, 3 * (size_t)s->max_batch))) {
