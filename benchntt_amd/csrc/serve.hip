@@ -328,10 +328,11 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
     std::vector<DevDesc> descs;
     U128 *arena = reinterpret_cast<U128 *>(s->arena_d);
     // a batch whose radix-8 items would leave resident workgroups idle runs
-    // 1024-value items: each level finishes in about half the time
+    // 1024-value items (each level finishes in about half the time), a
+    // batch that would not fill a quarter of them 512-value items
     size_t items8 = 0;
     for (const ReqView &r : reqs) items8 += items_per_request(r.L, 3);
-    const int want = items8 < 3 * (size_t)s->grid_cap ? 2 : 3;
+    const int want = items8 < (size_t)s->grid_cap ? 1 : items8 < 3 * (size_t)s->grid_cap ? 2 : 3;
     for (int L = s->min_l; L <= s->max_l; L++) {
         std::vector<DevDesc> fwd, inv;
         for (const ReqView &r : reqs) {
@@ -449,7 +450,7 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
         return bntt_set_error(BNTT_ENOMEM, "hipMalloc of the serving arena failed");
     }
     s->arena.reset(units);
-    const size_t cap_items = (size_t)s->max_batch * items_per_request(s->max_l, 2) + 64;
+    const size_t cap_items = (size_t)s->max_batch * items_per_request(s->max_l, 1) + 64;
     s->slots.resize(s->nslots + 1);
     for (auto &sl : s->slots)
         if ((rc = make_slot(sl, cap_items, 3 * (size_t)s->max_batch))) {
