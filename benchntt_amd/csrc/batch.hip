@@ -26,7 +26,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -155,6 +157,42 @@ void free_plan(bntt_batch_plan *p)
     delete p;
 }
 
+// A byte range one task reads or writes.
+struct Span {
+    uintptr_t lo, hi;  // [lo, hi)
+    size_t task;
+    bool write;
+};
+
+// Whether some task writes bytes that another task reads or writes (sweep over
+// spans by start; active spans are counted per task, all and writes only).
+bool cross_task_overlap(std::vector<Span> spans)
+{
+    std::sort(spans.begin(), spans.end(), [](const Span &a, const Span &b) { return a.lo < b.lo; });
+    typedef std::pair<uintptr_t, size_t> End;  // (hi, index)
+    std::priority_queue<End, std::vector<End>, std::greater<End>> ends;
+    std::map<size_t, size_t> active, active_w;  // task -> spans still open
+    auto drop = [](std::map<size_t, size_t> &m, size_t t) {
+        auto it = m.find(t);
+        if (--it->second == 0) m.erase(it);
+    };
+    for (size_t i = 0; i < spans.size(); i++) {
+        const Span &cur = spans[i];
+        while (!ends.empty() && ends.top().first <= cur.lo) {
+            const Span &old = spans[ends.top().second];
+            drop(active, old.task);
+            if (old.write) drop(active_w, old.task);
+            ends.pop();
+        }
+        const std::map<size_t, size_t> &against = cur.write ? active : active_w;
+        if (against.size() > 1 || (against.size() == 1 && against.begin()->first != cur.task)) return true;
+        active[cur.task]++;
+        if (cur.write) active_w[cur.task]++;
+        ends.push(End(cur.hi, i));
+    }
+    return false;
+}
+
 int launch_levels(bntt_batch_plan *p, hipStream_t s)
 {
     if (p->ntwjobs) {
@@ -188,7 +226,8 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     if (rc) return rc;
     if (!out || !m || !mu) return bntt_set_error(BNTT_EINVAL, "null pointer argument");
     *out = nullptr;
-    if ((ntasks && !tasks) || (nmuls && !muls)) return bntt_set_error(BNTT_EINVAL, "null task list");
+    if ((ntasks && !tasks) || (nmuls && !muls) || (ntune && !tune))
+        return bntt_set_error(BNTT_EINVAL, "null task list");
     if (ntasks > 0xffffffffu || nmuls > 0xffffffffu) return bntt_set_error(BNTT_EINVAL, "too many tasks");
 
     TuneChoice choice[bntt::kMaxLog2 + 1];
@@ -374,13 +413,53 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         }
     }
 
-    // products: level after the last pass of the transforms they read
-    std::unordered_map<const void *, int> level_of_output;
-    for (size_t j = 0; j < ntasks; j++) {
-        auto it = level_of_output.find(tasks[j].y);
-        const int lv = out_level[j] + 1;
-        if (it == level_of_output.end() || it->second < lv) level_of_output[tasks[j].y] = lv;
+    // transforms are independent: none may write what another reads or writes;
+    // products may read transform outputs and write anything no transform uses
+    {
+        std::vector<Span> spans;
+        spans.reserve(2 * ntasks + 3 * nmuls);
+        for (size_t j = 0; j < ntasks; j++) {
+            const uintptr_t bytes = sizeof(U128) << tasks[j].log2n;
+            const uintptr_t x = reinterpret_cast<uintptr_t>(tasks[j].x), y = reinterpret_cast<uintptr_t>(tasks[j].y);
+            spans.push_back(Span{x, x + bytes, j, false});
+            spans.push_back(Span{y, y + bytes, j, true});
+        }
+        for (size_t i = 0; i < nmuls; i++) {
+            const uintptr_t c = reinterpret_cast<uintptr_t>(muls[i].c);
+            spans.push_back(Span{c, c + sizeof(U128) * muls[i].n, ntasks + i, true});
+        }
+        std::vector<Span> prod;  // products among themselves: operands against results
+        prod.reserve(3 * nmuls);
+        for (size_t i = 0; i < nmuls; i++) {
+            const uintptr_t bytes = sizeof(U128) * muls[i].n;
+            const uintptr_t a = reinterpret_cast<uintptr_t>(muls[i].a), b = reinterpret_cast<uintptr_t>(muls[i].b),
+                            c = reinterpret_cast<uintptr_t>(muls[i].c);
+            prod.push_back(Span{a, a + bytes, i, false});
+            prod.push_back(Span{b, b + bytes, i, false});
+            prod.push_back(Span{c, c + bytes, i, true});
+        }
+        if (cross_task_overlap(spans) || cross_task_overlap(prod)) {
+            free_plan(p);
+            return bntt_set_error(BNTT_EINVAL, "a task writes memory another task reads or writes");
+        }
     }
+    // products: level after the last pass of every transform whose output
+    // overlaps one of their operands
+    std::map<uintptr_t, std::pair<uintptr_t, int>> outputs;  // y -> (end, level after it)
+    for (size_t j = 0; j < ntasks; j++) {
+        const uintptr_t y = reinterpret_cast<uintptr_t>(tasks[j].y);
+        outputs[y] = std::make_pair(y + (sizeof(U128) << tasks[j].log2n), out_level[j] + 1);
+    }
+    auto ready_level = [&](const void *a, uint64_t n) {
+        const uintptr_t lo = reinterpret_cast<uintptr_t>(a), hi = lo + sizeof(U128) * n;
+        int lvl = 0;
+        // outputs are disjoint: those overlapping [lo, hi) start below hi and end above lo
+        auto it = outputs.upper_bound(lo);
+        if (it != outputs.begin()) --it;
+        for (; it != outputs.end() && it->first < hi; ++it)
+            if (it->second.first > lo) lvl = std::max(lvl, it->second.second);
+        return lvl;
+    };
     std::vector<DevMul> mv(nmuls);
     for (size_t i = 0; i < nmuls; i++) {
         const bntt_mul_task &t = muls[i];
@@ -392,11 +471,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         }
         mv[i] = DevMul{static_cast<const U128 *>(t.a), static_cast<const U128 *>(t.b), static_cast<U128 *>(t.c),
                        t.n};
-        int lvl = 0;
-        for (const void *src : {t.a, t.b}) {
-            const auto it = level_of_output.find(src);
-            if (it != level_of_output.end()) lvl = std::max(lvl, it->second);
-        }
+        const int lvl = std::max(ready_level(t.a, t.n), ready_level(t.b, t.n));
         for (uint64_t off = 0; off < t.n; off += bntt::kQueueTile) {
             WorkItem w{};
             w.kind = bntt::ITEM_MUL;

@@ -106,8 +106,10 @@ typedef struct bntt_ntt_task {
 } bntt_ntt_task;
 
 /* One pointwise product c[i] = a[i] * b[i] mod m, i < n (device pointers).
- * When a or b is the y of a transform in the same plan, the product runs
- * after that transform. */
+ * When a or b overlaps the y of transforms in the same plan, the product runs
+ * after those transforms.  Tasks of a plan are independent: a transform's y
+ * may not overlap another transform's x or y, a product's c may not overlap
+ * any transform's buffers or another product's operands (BNTT_EINVAL). */
 typedef struct bntt_mul_task {
     const void *a;
     const void *b;

@@ -175,3 +175,43 @@ def test_uneven_20000_full_check(gpu, orc):
     stream.synchronize()
     check_all(orc, wl, from_dev(dy), from_dev(dc), threads=16)
     plan.close()
+
+
+def test_product_on_part_of_an_output(gpu, orc):
+    """A product reading the middle of a two-pass transform's output (not its
+    start) runs after that transform's last pass."""
+    from benchntt_amd.batch import BatchPlan
+
+    L = 16
+    x, tw = uniform(1 << L, QN, 90), uniform(1 << L, QN, 91)
+    x2, tw2 = uniform(1 << L, QN, 92), uniform(1 << L, QN, 93)
+    dx, dtw, dx2, dtw2 = (to_dev(gpu, a) for a in (x, tw, x2, tw2))
+    dy, dy2 = gpu.empty_like(dx), gpu.empty_like(dx2)
+    dc = gpu.empty((4096, 2), dtype=gpu.int64, device="cuda")
+    plan = BatchPlan([(dx, dy, dtw), (dx2, dy2, dtw2)], [(dy[5000:9096], dy2[12345:16441], dc)], QN, MUN,
+                     graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    y, y2 = orc.ntt(x, tw, QN, MUN), orc.ntt(x2, tw2, QN, MUN)
+    assert np.array_equal(from_dev(dc), orc.vvmul(y[5000:9096], y2[12345:16441], QN, MUN))
+    plan.close()
+
+
+def test_plan_rejects_overlapping_tasks(gpu):
+    from benchntt_amd import BnttError
+    from benchntt_amd.batch import BatchPlan
+
+    a = to_dev(gpu, uniform(4096, QN, 1))
+    tw = to_dev(gpu, uniform(1024, QN, 2))
+    out = gpu.empty((4096, 2), dtype=gpu.int64, device="cuda")
+    with pytest.raises(BnttError):  # the second transform writes what the first reads
+        BatchPlan([(a[:1024], out[:1024], tw), (a[1024:2048], a[512:1536], tw)], [], QN, MUN)
+    with pytest.raises(BnttError):  # two transforms write overlapping outputs
+        BatchPlan([(a[:1024], out[:1024], tw), (a[1024:2048], out[512:1536], tw)], [], QN, MUN)
+    with pytest.raises(BnttError):  # a product writes a transform's input
+        BatchPlan([(a[:1024], out[:1024], tw)], [(out[:1024], out[:1024], a[:1024])], QN, MUN)
+    with pytest.raises(BnttError):  # a product writes another product's operand
+        BatchPlan([], [(a[:512], a[:512], out[:512]), (out[256:768], a[:512], out[1024:1536])], QN, MUN)
+    # in place, and products reading outputs: accepted
+    BatchPlan([(a[:1024], a[:1024], tw), (a[1024:2048], out[:1024], tw)],
+              [(out[:1024], a[:1024], out[2048:3072])], QN, MUN).close()
