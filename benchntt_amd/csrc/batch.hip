@@ -28,7 +28,6 @@
 #include <cstring>
 #include <map>
 #include <mutex>
-#include <queue>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -38,6 +37,7 @@
 #include "ntt_core.hpp"
 
 #include "queue.hpp"
+#include "spans.hpp"
 
 namespace bntt {
 
@@ -155,42 +155,6 @@ void free_plan(bntt_batch_plan *p)
     (void)hipFree(p->twjobs_d);
     (void)hipFree(p->twchunks_d);
     delete p;
-}
-
-// A byte range one task reads or writes.
-struct Span {
-    uintptr_t lo, hi;  // [lo, hi)
-    size_t task;
-    bool write;
-};
-
-// Whether some task writes bytes that another task reads or writes (sweep over
-// spans by start; active spans are counted per task, all and writes only).
-bool cross_task_overlap(std::vector<Span> spans)
-{
-    std::sort(spans.begin(), spans.end(), [](const Span &a, const Span &b) { return a.lo < b.lo; });
-    typedef std::pair<uintptr_t, size_t> End;  // (hi, index)
-    std::priority_queue<End, std::vector<End>, std::greater<End>> ends;
-    std::map<size_t, size_t> active, active_w;  // task -> spans still open
-    auto drop = [](std::map<size_t, size_t> &m, size_t t) {
-        auto it = m.find(t);
-        if (--it->second == 0) m.erase(it);
-    };
-    for (size_t i = 0; i < spans.size(); i++) {
-        const Span &cur = spans[i];
-        while (!ends.empty() && ends.top().first <= cur.lo) {
-            const Span &old = spans[ends.top().second];
-            drop(active, old.task);
-            if (old.write) drop(active_w, old.task);
-            ends.pop();
-        }
-        const std::map<size_t, size_t> &against = cur.write ? active : active_w;
-        if (against.size() > 1 || (against.size() == 1 && against.begin()->first != cur.task)) return true;
-        active[cur.task]++;
-        if (cur.write) active_w[cur.task]++;
-        ends.push(End(cur.hi, i));
-    }
-    return false;
 }
 
 int launch_levels(bntt_batch_plan *p, hipStream_t s)
@@ -416,29 +380,29 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     // transforms are independent: none may write what another reads or writes;
     // products may read transform outputs and write anything no transform uses
     {
-        std::vector<Span> spans;
+        std::vector<bntt::Span> spans;
         spans.reserve(2 * ntasks + 3 * nmuls);
         for (size_t j = 0; j < ntasks; j++) {
             const uintptr_t bytes = sizeof(U128) << tasks[j].log2n;
             const uintptr_t x = reinterpret_cast<uintptr_t>(tasks[j].x), y = reinterpret_cast<uintptr_t>(tasks[j].y);
-            spans.push_back(Span{x, x + bytes, j, false});
-            spans.push_back(Span{y, y + bytes, j, true});
+            spans.push_back(bntt::Span{x, x + bytes, j, false});
+            spans.push_back(bntt::Span{y, y + bytes, j, true});
         }
         for (size_t i = 0; i < nmuls; i++) {
             const uintptr_t c = reinterpret_cast<uintptr_t>(muls[i].c);
-            spans.push_back(Span{c, c + sizeof(U128) * muls[i].n, ntasks + i, true});
+            spans.push_back(bntt::Span{c, c + sizeof(U128) * muls[i].n, ntasks + i, true});
         }
-        std::vector<Span> prod;  // products among themselves: operands against results
+        std::vector<bntt::Span> prod;  // products among themselves: operands against results
         prod.reserve(3 * nmuls);
         for (size_t i = 0; i < nmuls; i++) {
             const uintptr_t bytes = sizeof(U128) * muls[i].n;
             const uintptr_t a = reinterpret_cast<uintptr_t>(muls[i].a), b = reinterpret_cast<uintptr_t>(muls[i].b),
                             c = reinterpret_cast<uintptr_t>(muls[i].c);
-            prod.push_back(Span{a, a + bytes, i, false});
-            prod.push_back(Span{b, b + bytes, i, false});
-            prod.push_back(Span{c, c + bytes, i, true});
+            prod.push_back(bntt::Span{a, a + bytes, i, false});
+            prod.push_back(bntt::Span{b, b + bytes, i, false});
+            prod.push_back(bntt::Span{c, c + bytes, i, true});
         }
-        if (cross_task_overlap(spans) || cross_task_overlap(prod)) {
+        if (bntt::cross_task_overlap(spans) || bntt::cross_task_overlap(prod)) {
             free_plan(p);
             return bntt_set_error(BNTT_EINVAL, "a task writes memory another task reads or writes");
         }

@@ -40,6 +40,7 @@
 #include "internal.hpp"
 #include "hostmont.hpp"
 #include "queue.hpp"
+#include "spans.hpp"
 
 using bntt::DevDesc;
 using bntt::Mod;
@@ -394,6 +395,24 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
     return BNTT_OK;
 }
 
+// A request's c may not overlap another request's operands or result.
+int check_disjoint(const bntt_pm_request *reqs, size_t n)
+{
+    std::vector<bntt::Span> spans;
+    spans.reserve(3 * n);
+    for (size_t i = 0; i < n; i++) {
+        const uintptr_t bytes = sizeof(U128) << reqs[i].log2n;
+        const uintptr_t a = reinterpret_cast<uintptr_t>(reqs[i].a), b = reinterpret_cast<uintptr_t>(reqs[i].b),
+                        c = reinterpret_cast<uintptr_t>(reqs[i].c);
+        spans.push_back(bntt::Span{a, a + bytes, i, false});
+        spans.push_back(bntt::Span{b, b + bytes, i, false});
+        spans.push_back(bntt::Span{c, c + bytes, i, true});
+    }
+    if (bntt::cross_task_overlap(spans))
+        return bntt_set_error(BNTT_EINVAL, "a request's result overlaps another request's operands or result");
+    return BNTT_OK;
+}
+
 int check_request(const bntt_pm_server *s, const bntt_pm_request &r)
 {
     if (r.log2n < s->min_l || r.log2n > s->max_l) return bntt_set_error(BNTT_EINVAL, "request size not served");
@@ -495,9 +514,10 @@ int bntt_pm_multiply(bntt_pm_server *s, const bntt_pm_request *reqs, size_t n)
     std::vector<ReqView> views;
     std::vector<std::pair<size_t, size_t>> allocs;
     int rc = BNTT_OK;
+    for (size_t i = 0; i < n; i++)
+        if ((rc = check_request(s, reqs[i]))) return rc;
+    if ((rc = check_disjoint(reqs, n))) return rc;
     for (size_t i = 0; i < n && !rc; i++) {
-        rc = check_request(s, reqs[i]);
-        if (rc) break;
         size_t off;
         const size_t u = request_units(reqs[i].log2n);
         if (!s->arena.alloc(u, &off)) {
@@ -525,6 +545,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
         if (rc) return rc;
         if (i && arrival_ns[i] < arrival_ns[i - 1]) return bntt_set_error(BNTT_EINVAL, "arrivals must be sorted");
     }
+    if (const int rc = check_disjoint(reqs, n)) return rc;
     bntt_pm_stats st{};
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();

@@ -178,7 +178,8 @@ int bntt_ntt_launch_each_dev(const bntt_ntt_task *tasks, size_t ntasks, const bn
  * server runs on streams of its own that do not wait for other streams:
  * operands must be in device memory before a call, and c may be read by any
  * stream after it returns.  A result does not depend on the other requests of
- * its batch.
+ * its batch.  Within one call, a request's c may not overlap another
+ * request's a, b or c (BNTT_EINVAL); c may equal its own a or b.
  */
 typedef struct bntt_pm_request {
     const void *a; /* N residues */

@@ -198,5 +198,9 @@ def test_errors(gpu):
         srv.multiply([(a[:100], a[:100], a[:100].clone())])  # not a power of two
     ok = to_dev(gpu, rand(1 << 10, q, 2))
     out = gpu.empty_like(ok)
+    other = to_dev(gpu, rand(1 << 10, q, 3))
+    with pytest.raises(BnttError):  # one request's result is another's operand
+        srv.multiply([(ok, ok, out), (out, other, gpu.empty_like(ok))])
     srv.multiply([(ok, ok, out)])  # the server still works after the errors
+    srv.multiply([(other, ok, other)])  # a result may replace its own operand
     srv.close()
