@@ -27,24 +27,31 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     // block's data is one contiguous range.  Otherwise element T_idx of group g
     // sits at transform*N + T_idx*2^mlog + M; consecutive threads take
     // consecutive g (consecutive addresses).
+    if (mlog == 0) {
+        // the workgroup's G transforms are one contiguous range
+        const U128 *xb = x + (gamma0 << L);
 #pragma unroll
-    for (int j = 0; j < E; j++) {
-        const int e = threadIdx.x + j * T;
-        int g, ti;
-        if (mlog == 0) {
-            g = e >> k;
-            ti = e & ((1 << k) - 1);
-        } else {
-            g = e & (G - 1);
-            ti = e >> Glog;
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * T;
+            if ((e >> k) < nvalid) {
+                const U128 a = ld16(xb + e);
+                big |= is_big(a);
+                lds[lidx(e, k)] = a;
+            }
         }
-        if (g < nvalid) {
-            const size_t gamma = gamma0 + (size_t)g;
-            const size_t tr = gamma >> mlog;
-            const size_t M = gamma & (((size_t)1 << mlog) - 1);
-            const U128 a = ld16(x + (tr << L) + ((size_t)ti << mlog) + M);
-            big |= is_big(a);
-            lds[lidx((g << k) + ti, k)] = a;
+    } else {
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * T;
+            const int g = e & (G - 1), ti = e >> Glog;
+            if (g < nvalid) {
+                const size_t gamma = gamma0 + (size_t)g;
+                const size_t tr = gamma >> mlog;
+                const size_t M = gamma & (((size_t)1 << mlog) - 1);
+                const U128 a = ld16(x + (tr << L) + ((size_t)ti << mlog) + M);
+                big |= is_big(a);
+                lds[lidx((g << k) + ti, k)] = a;
+            }
         }
     }
 
@@ -69,11 +76,11 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     }
 
     // LDS -> HBM: the G groups' outputs are one contiguous range.
-    const size_t out0 = gamma0 << k;
+    U128 *yb = y + (gamma0 << k);
 #pragma unroll
     for (int j = 0; j < E; j++) {
         const int e = threadIdx.x + j * T;
-        if ((e >> k) < nvalid) st16(y + out0 + e, lds[lidx(e, k)]);
+        if ((e >> k) < nvalid) st16(yb + e, lds[lidx(e, k)]);
     }
 }
 

@@ -314,6 +314,16 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
     bool big = false;
     const int ntw = (int)(((1u << k) - 1u) << wlog);
     const int nstaged = (int)(((1u << (klds < k ? klds : k)) - 1u) << wlog);  // stages r < klds come first
+    if (S == 0) {
+        // the first stages of a transform (one prefix): the staged tables are
+        // the table itself from entry 1 on
+        for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
+            const U128 a = ld16(tw + 1 + e);
+            big |= is_big(a);
+            if (e < nstaged) tws[tidx(e)] = to_r31(a);
+        }
+        return big;
+    }
     for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
         // e = W*(2^r - 1) + rel*2^r + j  ->  stage r, offset within the stage table
         const uint32_t u = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
