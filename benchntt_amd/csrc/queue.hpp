@@ -185,11 +185,11 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         } else {
             const int g = e & (G - 1), ti = e >> Glog;
             if (g < nvalid) {
-                const size_t M = it.first + (size_t)g;
-                const size_t idx = ((size_t)ti << mlog) + M;
+                // indices within one transform (at most 2^24 points) fit 32 bits
+                const uint32_t idx = ((uint32_t)ti << mlog) + (uint32_t)it.first + (uint32_t)g;
                 U128 a;
                 if (gather) {
-                    const uint32_t r = bitrev((uint32_t)idx, (uint32_t)L);
+                    const uint32_t r = bitrev(idx, (uint32_t)L);
                     a = montmul_r31(ld16(d0.x + r), to_r31(ld16(d0.x2 + r)), md.mq);
                 } else {
                     a = ld16(src + idx);
@@ -259,9 +259,9 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             }
         } else {
             if ((e >> k) < nvalid) {
-                const size_t idx = (it.first << k) + (size_t)e;
+                const uint32_t idx = ((uint32_t)it.first << k) + (uint32_t)e;
                 if (scatter)
-                    st16(dst + bitrev((uint32_t)idx, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[idx], md.mq));
+                    st16(dst + bitrev(idx, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[idx], md.mq));
                 else
                     st16(dst + idx, lds[lidx(e, k)]);
             }
