@@ -542,7 +542,12 @@ int bntt_batch_plan_execute(bntt_batch_plan *p, void *stream)
     if (!p) return bntt_set_error(BNTT_EINVAL, "null plan");
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(p->mu);
-    if (!p->use_graph || s == nullptr) return launch_levels(p, s);
+    // inside the caller's own stream capture the launches become part of the
+    // caller's graph
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (s != nullptr && hipStreamIsCapturing(s, &cs) != hipSuccess)
+        return bntt_set_error(BNTT_EHIP, "hipStreamIsCapturing");
+    if (!p->use_graph || s == nullptr || cs != hipStreamCaptureStatusNone) return launch_levels(p, s);
     if (!p->exec) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamBeginCapture");

@@ -215,3 +215,27 @@ def test_plan_rejects_overlapping_tasks(gpu):
     # in place, and products reading outputs: accepted
     BatchPlan([(a[:1024], a[:1024], tw), (a[1024:2048], out[:1024], tw)],
               [(out[:1024], a[:1024], out[2048:3072])], QN, MUN).close()
+
+
+@pytest.mark.parametrize("plan_graph", [False, True])
+def test_capture_in_caller_graph(gpu, orc, plan_graph):
+    """plan.execute() captured in the caller's own graph: every replay, with
+    the inputs refilled eagerly between replays, matches the oracle."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(24, QN, seed=21, lo=4, hi=14)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=plan_graph)
+    s = gpu.cuda.Stream()
+    s.wait_stream(gpu.cuda.current_stream())
+    g = gpu.cuda.CUDAGraph()
+    with gpu.cuda.graph(g, stream=s):
+        plan.execute()
+    for rep in range(3):
+        wl.x[:] = uniform(wl.x.shape[0], QN, 300 + rep)
+        dx.copy_(to_dev(gpu, wl.x))
+        gpu.cuda.synchronize()
+        g.replay()
+        gpu.cuda.synchronize()
+        check_all(orc, wl, from_dev(dy), from_dev(dc))
+    plan.close()
