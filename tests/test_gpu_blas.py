@@ -114,3 +114,34 @@ def test_errors(gpu):
     mis = raw[8:].view(gpu.int64)  # 8-byte aligned, not 16
     with pytest.raises(bn.BnttError):
         bn.vector_add(mis, mis, mis, QB)
+
+
+def test_host_entry_points_from_many_threads(gpu, orc):
+    """The host-pointer entry points are thread-safe: 8 threads call axpy,
+    vvmul and ntt on host arrays at once (shared staging buffers), each result
+    matches the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import benchntt_amd as bn
+
+    def work(i):
+        n = 1000 + 517 * i
+        a, b = uniform(n, C.Q_BLAS, 40 + i), uniform(n, C.Q_BLAS, 80 + i)
+        c = np.empty_like(a)
+        bn.axpy(c, a, b, C.AXPY_SCALAR, C.Q_BLAS, C.MU_BLAS)
+        if not np.array_equal(c, orc.axpy(a, b, C.AXPY_SCALAR, C.Q_BLAS, C.MU_BLAS)):
+            return f"axpy thread {i}"
+        bn.vector_mul(c, a, b, C.Q_BLAS, C.MU_BLAS)
+        if not np.array_equal(c, orc.vvmul(a, b, C.Q_BLAS, C.MU_BLAS)):
+            return f"vvmul thread {i}"
+        L = 8 + (i % 7)
+        x, tw = uniform(1 << L, C.Q_NTT, 120 + i), uniform(1 << L, C.Q_NTT, 160 + i)
+        y = np.empty_like(x)
+        bn.ntt(y, x, C.Q_NTT, C.MU_NTT, tw)
+        if not np.array_equal(y, orc.ntt(x, tw, C.Q_NTT, C.MU_NTT)):
+            return f"ntt thread {i}"
+        return None
+
+    with ThreadPoolExecutor(8) as ex:
+        errs = [e for r in range(3) for e in ex.map(work, range(8 * r, 8 * r + 8)) if e]
+    assert not errs, errs
