@@ -341,6 +341,12 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             w.desc = (uint32_t)packed.size();
             w.nvalid = (uint32_t)(i1 - i0);
             w.first = 0;
+            // transforms back to back in x and in y: one base pointer each
+            bool contig = true;
+            for (size_t j = i0 + 1; j < i1 && contig; j++)
+                contig = descs[order[j]].x == descs[order[i0]].x + ((j - i0) << L) &&
+                         descs[order[j]].y == descs[order[i0]].y + ((j - i0) << L);
+            if (contig) w.flags = bntt::IT_CONTIG;
             for (size_t j = i0; j < i1; j++) {
                 packed.push_back(descs[order[j]]);
                 out_level[order[j]] = 0;

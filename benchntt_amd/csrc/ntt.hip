@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     tv.glob = tw;
     tv.W = W;
     if (!slow) {
-        if constexpr (PM) run_group_network<ELOG, TW_LDS, PM, true>(lds, tv, k, S, mbase, md, klds);
+        if constexpr (PM) run_group_network<ELOG, TW_LDS, PM, TW_G128>(lds, tv, k, S, mbase, md, klds);
     } else {
         run_group_network<ELOG, TW_SLOW, PM>(lds, tv, k, S, mbase, md);
     }

@@ -261,21 +261,23 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
     __syncthreads();
 }
 
-// A TW_LDS network whose tables are staged only for its first klds stages
-// (PARTIAL) runs the sub-passes that reach beyond them with TW_G128.
-template <int ELOG, int MODE, bool PM, bool PARTIAL>
+// A network whose tables are staged in LDS (MODE = TW_LDS or TW_MONT_LDS)
+// only for its first klds stages runs the sub-passes that reach beyond them
+// with FALLBACK (TW_G128 or TW_GLOBAL, or TW_MONT for TW_MONT_LDS); FALLBACK
+// = -1: every sub-pass uses MODE.
+template <int ELOG, int MODE, bool PM, int FALLBACK>
 __device__ __forceinline__ void run_subpass(U128 *lds, const TwView &tv, int k, int S, int r0, int kp, bool lds_tw,
                                             uint32_t mbase, const Mod &md)
 {
-    if constexpr (MODE == TW_LDS && PARTIAL) {
+    if constexpr (FALLBACK >= 0) {
         if (!lds_tw) {
             if (kp == ELOG) {
-                lds_subpass<ELOG, ELOG, TW_G128, PM>(lds, tv, k, S, r0, mbase, md);
+                lds_subpass<ELOG, ELOG, FALLBACK, PM>(lds, tv, k, S, r0, mbase, md);
             } else if constexpr (ELOG >= 2) {
                 if (kp == 1) {
-                    lds_subpass<ELOG, 1, TW_G128, PM>(lds, tv, k, S, r0, mbase, md);
+                    lds_subpass<ELOG, 1, FALLBACK, PM>(lds, tv, k, S, r0, mbase, md);
                 } else if constexpr (ELOG >= 3) {
-                    if (kp == 2) lds_subpass<ELOG, 2, TW_G128, PM>(lds, tv, k, S, r0, mbase, md);
+                    if (kp == 2) lds_subpass<ELOG, 2, FALLBACK, PM>(lds, tv, k, S, r0, mbase, md);
                 }
             }
             return;
@@ -292,14 +294,14 @@ __device__ __forceinline__ void run_subpass(U128 *lds, const TwView &tv, int k, 
     }
 }
 
-template <int ELOG, int MODE, bool PM, bool PARTIAL = false>
+template <int ELOG, int MODE, bool PM, int FALLBACK = -1>
 __device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase,
                                                   const Mod &md, int klds = 32)
 {
     int r0 = 0;
     while (r0 < k) {
         const int kp = (k - r0) < ELOG ? (k - r0) : ELOG;
-        run_subpass<ELOG, MODE, PM, PARTIAL>(lds, tv, k, S, r0, kp, r0 + kp <= klds, mbase, md);
+        run_subpass<ELOG, MODE, PM, FALLBACK>(lds, tv, k, S, r0, kp, r0 + kp <= klds, mbase, md);
         r0 += kp;
     }
 }
@@ -338,10 +340,14 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
 
 // The same staging from a table already in digits (no check: the caller
 // knows whether the table is small).
-__device__ __forceinline__ void stage_twiddles_r31(R31 *tws, const R31 *__restrict__ twr, int S, int k, int wlog,
+__device__ __forceinline__ void stage_twiddles_r31(R31 *tws, const R31 *__restrict__ twr, int S, int klds, int wlog,
                                                    uint32_t P0start)
 {
-    const int ntw = (int)(((1u << k) - 1u) << wlog);
+    const int ntw = (int)(((1u << klds) - 1u) << wlog);  // stages [S, S + klds)
+    if (S == 0) {
+        for (int e = threadIdx.x; e < ntw; e += blockDim.x) tws[tidx(e)] = twr[1 + e];
+        return;
+    }
     for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
         const uint32_t u = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
         const int r = 31 - __builtin_clz(u);

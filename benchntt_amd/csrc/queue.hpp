@@ -13,7 +13,9 @@
 // steps of a negacyclic product into its first load and its last store:
 //   IT_GATHER_MUL     the first pass reads x[i] = A[brv(i)] * B[brv(i)] * R^-1
 //   IT_SCATTER_SCALE  the last pass writes C[brv(i)] = y[i] * S[i] * R^-1
-// where brv reverses the log2 N index bits of a transform.
+// where brv reverses the log2 N index bits of a transform.  IT_CONTIG marks a
+// packed item whose transforms lie back to back in x and in y: it is read and
+// written through the first descriptor's pointers.
 #pragma once
 
 #include "ntt_core.hpp"
@@ -30,7 +32,7 @@ constexpr size_t kQueueMaxLds = 80 * 1024;
 constexpr size_t kQueueLds3 = 46 * 1024;
 
 enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
-enum ItemFlags : uint8_t { IT_GATHER_MUL = 1, IT_SCATTER_SCALE = 2 };
+enum ItemFlags : uint8_t { IT_GATHER_MUL = 1, IT_SCATTER_SCALE = 2, IT_CONTIG = 4 };
 
 struct DevDesc {
     const U128 *x;
@@ -80,20 +82,22 @@ struct TwChunk {
 };
 constexpr int kTwChunk = 4096;
 
+// klds > 0: the tables of the first klds stages are staged in LDS, the later
+// stages read the digit table in device memory.
 template <int ELOG, bool MONT>
 __device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, bool slow,
-                                                 bool staged, const Mod &md)
+                                                 int klds, const Mod &md)
 {
     if constexpr (MONT) {
-        if (staged)
-            run_group_network<ELOG, TW_MONT_LDS, true>(lds, tv, k, S, mbase, md);
+        if (klds > 0)
+            run_group_network<ELOG, TW_MONT_LDS, true, TW_MONT>(lds, tv, k, S, mbase, md, klds);
         else
             run_group_network<ELOG, TW_MONT, true>(lds, tv, k, S, mbase, md);
     } else {
         if (slow)
             run_group_network<ELOG, TW_SLOW, true>(lds, tv, k, S, mbase, md);
-        else if (staged)
-            run_group_network<ELOG, TW_LDS, true>(lds, tv, k, S, mbase, md);
+        else if (klds > 0)
+            run_group_network<ELOG, TW_LDS, true, TW_GLOBAL>(lds, tv, k, S, mbase, md, klds);
         else
             run_group_network<ELOG, TW_GLOBAL, true>(lds, tv, k, S, mbase, md);
     }
@@ -114,17 +118,20 @@ __host__ __device__ inline int item_tw_wlog(int L, int k, int S, int Glog)
 }
 
 // Layout of a launch over items whose data tiles need at most data_slots and
-// whose staged tables need tw_needs[i] digit slots: the twiddle area holds the
-// largest need that fits `budget` bytes of dynamic LDS (items needing more read
-// their twiddles from device memory).  Returns the dynamic LDS bytes.
+// whose full staged tables need tw_needs[i] digit slots: the twiddle area gets
+// what `budget` bytes of dynamic LDS leave beside the data, at most the
+// largest need (an item stages the tables of as many of its first stages as
+// fit).  Returns the dynamic LDS bytes.
 inline size_t plan_tw_area(size_t data_slots, const size_t *tw_needs, size_t n, size_t budget, TwArea *area)
 {
-    size_t cap = 0;
-    for (size_t i = 0; i < n; i++)
-        if (tw_needs[i] > cap && sizeof(U128) * data_slots + sizeof(R31) * tw_needs[i] <= budget) cap = tw_needs[i];
+    size_t need = 0;
+    for (size_t i = 0; i < n; i++) need = tw_needs[i] > need ? tw_needs[i] : need;
+    const size_t data = sizeof(U128) * data_slots;
+    size_t cap = budget > data ? (budget - data) / sizeof(R31) : 0;
+    if (cap > need) cap = need;
     area->tw_off = (uint32_t)data_slots;
     area->tw_cap = (uint32_t)cap;
-    return sizeof(U128) * data_slots + sizeof(R31) * cap;
+    return data + sizeof(R31) * cap;
 }
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
@@ -156,7 +163,8 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     // fused pointwise steps (Montgomery products only)
     const bool gather = MONT && (it.flags & IT_GATHER_MUL) && it.pass == 0;
     const bool scatter = MONT && (it.flags & IT_SCATTER_SCALE) && it.pass + 1 == it.np;
-    if (mlog == 0) {
+    const bool contig = !MONT && (it.flags & IT_CONTIG);
+    if (mlog == 0 && !contig) {
         // whole transforms, one per group, each with its own buffers
         if ((int)threadIdx.x < nvalid) {
             const DevDesc &d = descs[it.desc + threadIdx.x];
@@ -169,7 +177,13 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     // HBM -> LDS
     for (int j = 0; j < E; j++) {
         const int e = threadIdx.x + j * kQueueThreads;
-        if (mlog == 0) {
+        if (contig) {
+            if ((e >> k) < nvalid) {
+                const U128 a = ld16(d0.x + e);
+                big |= is_big(a);
+                lds[lidx(e, k)] = a;
+            }
+        } else if (mlog == 0) {
             const int g = e >> k, ti = e & ((1 << k) - 1);
             if (g < nvalid) {
                 U128 a;
@@ -218,13 +232,21 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     // a multi-pass item are consecutive M, the first a multiple of G)
     const int wlog = (mlog == 0) ? 0 : (Glog < S ? Glog : S);
     const uint32_t P0start = (mlog == 0 || S <= Glog) ? 0u : (uint32_t)(it.first & ((1ull << S) - 1ull));
-    const bool staged = !slow && area.tw_cap >= tw_slots(k, wlog);
-    if (staged) {
+    // stage the tables of as many first stages as the launch's area holds
+    // (at least one radix-8 sub-pass' worth)
+    int klds = 0;
+    if (!slow)
+        for (int j = k; j >= 3; j--)
+            if (area.tw_cap >= tw_slots(j, wlog)) {
+                klds = j;
+                break;
+            }
+    if (klds > 0) {
         R31 *tws = reinterpret_cast<R31 *>(lds + area.tw_off);
-        const uint64_t key = (uint64_t)S | ((uint64_t)k << 8) | ((uint64_t)wlog << 16) | ((uint64_t)P0start << 32);
+        const uint64_t key = (uint64_t)S | ((uint64_t)klds << 8) | ((uint64_t)wlog << 16) | ((uint64_t)P0start << 32);
         if (s_twkey[0] != reinterpret_cast<uint64_t>(d0.twr) || s_twkey[1] != key) {
             __syncthreads();  // every thread has read the key
-            stage_twiddles_r31(tws, d0.twr, S, k, wlog, P0start);
+            stage_twiddles_r31(tws, d0.twr, S, klds, wlog, P0start);
             if (threadIdx.x == 0) {
                 s_twkey[0] = reinterpret_cast<uint64_t>(d0.twr);
                 s_twkey[1] = key;
@@ -236,19 +258,21 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     }
     switch (elog) {
     case 1:
-        run_network_elog<1, MONT>(lds, tv, k, S, mbase, slow, staged, md);
+        run_network_elog<1, MONT>(lds, tv, k, S, mbase, slow, klds, md);
         break;
     case 2:
-        run_network_elog<2, MONT>(lds, tv, k, S, mbase, slow, staged, md);
+        run_network_elog<2, MONT>(lds, tv, k, S, mbase, slow, klds, md);
         break;
     default:
-        run_network_elog<3, MONT>(lds, tv, k, S, mbase, slow, staged, md);
+        run_network_elog<3, MONT>(lds, tv, k, S, mbase, slow, klds, md);
         break;
     }
     // LDS -> HBM
     for (int j = 0; j < E; j++) {
         const int e = threadIdx.x + j * kQueueThreads;
-        if (mlog == 0) {
+        if (contig) {
+            if ((e >> k) < nvalid) st16(d0.y + e, lds[lidx(e, k)]);
+        } else if (mlog == 0) {
             const int g = e >> k;
             if (g < nvalid) {
                 const uint32_t ti = (uint32_t)(e & ((1 << k) - 1));

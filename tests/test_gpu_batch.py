@@ -239,3 +239,29 @@ def test_capture_in_caller_graph(gpu, orc, plan_graph):
         gpu.cuda.synchronize()
         check_all(orc, wl, from_dev(dy), from_dev(dc))
     plan.close()
+
+
+def test_contiguous_packed_items(gpu, orc):
+    """Single-pass transforms back to back in memory are read through one
+    base pointer per item; a gap or a permutation falls back to per-transform
+    pointers.  Both against the oracle."""
+    from benchntt_amd.batch import BatchPlan
+
+    for L, cnt in ((6, 300), (10, 37), (11, 9)):
+        n = 1 << L
+        x = uniform(cnt * n + n, QN, 500 + L)
+        tw = uniform(n, QN, 600 + L)
+        dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+        dy = gpu.empty_like(dx)
+        # back to back, then one with a gap of one transform, then reversed order
+        layouts = [list(range(cnt)), list(range(cnt // 2)) + list(range(cnt // 2 + 1, cnt + 1)),
+                   list(reversed(range(cnt)))]
+        for lay in layouts:
+            tasks = [(dx[i * n:(i + 1) * n], dy[i * n:(i + 1) * n], dtw) for i in lay]
+            plan = BatchPlan(tasks, [], QN, MUN, graph=False)
+            plan.execute()
+            gpu.cuda.synchronize()
+            y = from_dev(dy)
+            for i in lay[:: max(1, cnt // 7)] + [lay[-1]]:
+                assert np.array_equal(y[i * n:(i + 1) * n], orc.ntt(x[i * n:(i + 1) * n], tw, QN, MUN)), (L, i)
+            plan.close()
