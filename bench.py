@@ -183,6 +183,9 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     ms1 = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), 200, 20, stream)
     out["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
     out["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
+    out["ntt_plain_1024_butterflies_per_s"] = round(5120 / (ms1 * 1e-3), 1)
+    # INT-op/s = butterflies/s x the network's VALU lane-instructions per butterfly
+    out["ntt_plain_1024_int_ops_per_s"] = round(5120 / (ms1 * 1e-3) * VALU_PER_BUTTERFLY, 1)
     tune = load_tune_table()
     total = 1 << 23
     xb = torch.from_numpy(uniform(total, qn, 5).view(np.int64)).to(dev)

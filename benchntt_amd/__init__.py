@@ -5,6 +5,11 @@ and vvadd / vvsub / vvmul / axpy mod q (blas_scalar.c:578-608) as hand-written
 HIP kernels behind the C ABI of include/bntt.h (libbntt.so, built in-tree by
 ``python benchntt_amd/build.py``).  Importing this package loads the library
 and raises ImportError when it is missing; there is no CPU fallback.
+
+Modules: ops (the reference's names on device tensors or host arrays), batch
+(uneven batches: work-queue plans, graph replay, tune table), polymul
+(negacyclic products with continuous batching), fixtures (text fixtures, word
+order, root tables, seeded residues), autotune, workloads, residues, config.
 """
 from . import config  # noqa: F401
 from ._lib import BnttError, U128, load as _load_lib  # noqa: F401
