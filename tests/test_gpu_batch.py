@@ -115,7 +115,7 @@ def test_large_sizes_three_passes(gpu, orc):
     from benchntt_amd.batch import BatchPlan
 
     tasks, xs, tws, ys = [], [], [], []
-    for l in (17, 18, 20):
+    for l in (17, 18, 20, 24):
         x, tw = uniform(1 << l, QN, l), uniform(1 << l, QN, 50 + l)
         dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
         dy = gpu.empty_like(dx)

@@ -48,8 +48,10 @@ def test_random_batches(gpu, orc, L):
     assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
 
 
-@pytest.mark.parametrize("L", [17, 18, 20])
+@pytest.mark.parametrize("L", [17, 18, 20, 22, 24])
 def test_large_plain(gpu, orc, L):
+    """Up to the largest size the entry points accept (2^24 points: three
+    passes through device memory)."""
     n = 1 << L
     x, tw = uniform(n, QN, L), uniform(n, QN, 2 * L)
     assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
