@@ -554,6 +554,19 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
     std::vector<char> waited(n, 0);
     size_t next = 0, completed = 0, inflight_reqs = 0;
     std::vector<ReqView> views;
+    // on a failure: wait for the batches still in flight, return their arena
+    // blocks and free the slots, so the server stays usable
+    auto abandon = [&](int rc) {
+        for (int k = 0; k < s->nslots; k++) {
+            Slot &sl = s->slots[k];
+            if (sl.busy) (void)hipStreamSynchronize(sl.stream);
+            for (auto &a : sl.allocs) s->arena.release(a.first, a.second);
+            sl.allocs.clear();
+            sl.reqs.clear();
+            sl.busy = false;
+        }
+        return rc;
+    };
     while (completed < n) {
         bool progress = false;
         const uint64_t t = now_ns();
@@ -564,7 +577,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
             if (!sl.busy) continue;
             const hipError_t q = hipEventQuery(sl.done);
             if (q == hipErrorNotReady) continue;
-            if (q != hipSuccess) return bntt_set_hip_error(q, "serving batch failed");
+            if (q != hipSuccess) return abandon(bntt_set_hip_error(q, "serving batch failed"));
             const uint64_t tdone = now_ns();
             for (size_t r : sl.reqs) done_ns[r] = tdone;
             for (auto &a : sl.allocs) s->arena.release(a.first, a.second);
@@ -597,7 +610,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
             }
             if (views.empty()) break;  // budget full: wait for a batch to retire
             const int rc = launch_batch(s, sl, views);
-            if (rc) return rc;
+            if (rc) return abandon(rc);
             sl.busy = true;
             st.batches++;
             inflight_reqs += views.size();
