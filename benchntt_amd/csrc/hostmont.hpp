@@ -33,6 +33,7 @@ struct HostMont {
         mq.q = to_r31(u128_to_limbs(q));
         mq.qinv = to_r31(u128_to_limbs(qinv));
         mq.ql = u128_to_limbs(q);
+        mq.ql2 = u128_to_limbs(q << 1);
         // R mod q and R^2 mod q by doubling
         u128h v = 1;
         for (int i = 0; i < 248; i++) {

@@ -85,8 +85,8 @@ __device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFF
 //              the table itself (cached in L1/L2).
 //   TW_SLOW:   the caller's table, literal reference arithmetic.
 //   TW_MONT:   like TW_GLOBAL, but the digit table holds tw * 2^124 mod q and
-//              the arithmetic is the exact Montgomery product with canonical
-//              add/sub (polynomial-multiply path, any odd q < 2^122).
+//              the arithmetic is the Montgomery product with lazy butterflies
+//              (values below 4q; polynomial-multiply path, any odd q < 2^122).
 //   TW_MONT_LDS: TW_MONT arithmetic with TW_LDS staged tables.
 //   TW_G128:   the caller's table, entries converted to digits in registers,
 //              fast arithmetic (stages left unstaged by a TW_LDS workgroup).
@@ -148,7 +148,10 @@ __device__ __forceinline__ void reg_network_fast(U128 (&v)[1 << KP], const R31 (
     }
 }
 
-// The same stages with Montgomery-form digit twiddles: montmul(v, tw*R) = v*tw.
+// The same stages with Montgomery-form digit twiddles, montmul(v, tw*R) =
+// v*tw, and lazy reduction (Harvey's butterfly): values stay below 4q; the
+// even input is brought below 2q, the product is below 2q, and the outputs
+// x + p and x + 2q - p are below 4q.  The caller makes the results canonical.
 template <int KP>
 __device__ __forceinline__ void reg_network_mont(U128 (&v)[1 << KP], const R31 (&t)[(1 << KP) - 1], const Mod &md)
 {
@@ -158,9 +161,10 @@ __device__ __forceinline__ void reg_network_mont(U128 (&v)[1 << KP], const R31 (
         U128 w[1 << KP];
 #pragma unroll
         for (int i = 0; i < H; i++) {
-            const U128 p = montmul_r31(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.mq);
-            w[2 * i] = addmod_canon(v[i], p, md.mq.ql);
-            w[2 * i + 1] = submod_canon(v[i], p, md.mq.ql);
+            const U128 x = sub_if_ge(v[i], md.mq.ql2);
+            const U128 p = montmul_r31_lazy(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.mq);
+            w[2 * i] = add128l(x, p);
+            w[2 * i + 1] = add128l(x, sub128l(md.mq.ql2, p));
         }
 #pragma unroll
         for (int i = 0; i < 2 * H; i++) v[i] = w[i];

@@ -13,6 +13,7 @@
 // steps of a negacyclic product into its first load and its last store:
 //   IT_GATHER_MUL     the first pass reads x[i] = A[brv(i)] * B[brv(i)] * R^-1
 //   IT_SCATTER_SCALE  the last pass writes C[brv(i)] = y[i] * S[i] * R^-1
+// (values between the Montgomery steps are kept below 4q; C is canonical)
 // where brv reverses the log2 N index bits of a transform.  IT_CONTIG marks a
 // packed item whose transforms lie back to back in x and in y: it is read and
 // written through the first descriptor's pointers.
@@ -134,6 +135,14 @@ inline size_t plan_tw_area(size_t data_slots, const size_t *tw_needs, size_t n, 
     return data + sizeof(R31) * cap;
 }
 
+// A * B * R^-1 for forward outputs A, B below 4q (lazy butterflies): B is
+// brought below q, the product is below 2q (a valid network input).
+__device__ __forceinline__ U128 gather_product(const U128 &a, const U128 &b, const MontQ &mq)
+{
+    const U128 bq = sub_if_ge(sub_if_ge(b, mq.ql2), mq.ql);
+    return montmul_r31_lazy(a, to_r31(bq), mq);
+}
+
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
 {
     return log2n ? (__brev(i) >> (32u - log2n)) : 0u;
@@ -189,7 +198,7 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                 U128 a;
                 if (gather) {
                     const uint32_t r = bitrev((uint32_t)ti, (uint32_t)L);
-                    a = montmul_r31(ld16(s_x[g] + r), to_r31(ld16(s_x2[g] + r)), md.mq);
+                    a = gather_product(ld16(s_x[g] + r), ld16(s_x2[g] + r), md.mq);
                 } else {
                     a = ld16(s_x[g] + ti);
                 }
@@ -204,7 +213,7 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                 U128 a;
                 if (gather) {
                     const uint32_t r = bitrev(idx, (uint32_t)L);
-                    a = montmul_r31(ld16(d0.x + r), to_r31(ld16(d0.x2 + r)), md.mq);
+                    a = gather_product(ld16(d0.x + r), ld16(d0.x2 + r), md.mq);
                 } else {
                     a = ld16(src + idx);
                 }

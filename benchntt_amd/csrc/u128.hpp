@@ -45,6 +45,7 @@ struct MontQ {
     R31 q;       // q in digits
     R31 qinv;    // -q^-1 mod 2^124 in digits
     U128 ql;     // q in 32-bit limbs
+    U128 ql2;    // 2q in 32-bit limbs
 };
 
 // Modulus descriptor passed to kernels by value.
@@ -316,7 +317,10 @@ BNTT_HD U128 mulmod_r31(const U128 &a_, const R31 &w, uint32_t c)
 // canonical: u = a*w + m*q with m = (a*w mod R)*qinv mod R is divisible by R
 // and u/R < (q^2 + R*q)/R < 2q, so one conditional subtraction finishes.
 
-BNTT_HD U128 montmul_r31(const U128 &a_, const R31 &w, const MontQ &mq)
+// Montgomery product a * w * 2^-124 mod q for odd q < 2^122, without the final
+// subtraction: for a < 4q and w < q the result is below 2q (T + m q < 4q^2 +
+// 2^124 q, and 4q < 2^124), which lazy butterflies keep as is.
+BNTT_HD U128 montmul_r31_lazy(const U128 &a_, const R31 &w, const MontQ &mq)
 {
     const R31 a = to_r31(a_);
     uint64_t t;
@@ -385,20 +389,57 @@ BNTT_HD U128 montmul_r31(const U128 &a_, const R31 &w, const MontQ &mq)
     t = mad32(m[3], q.d[3], (t >> 31) + d[6]);
     r[2] = (uint32_t)t & kM31;
     r[3] = (uint32_t)(t >> 31) + d[7];  // u/R < 2q < 2^123: fits, no carry out
-    // digits -> 32-bit limbs, then subtract q once if needed
-    const uint32_t w0 = r[0] | (r[1] << 31), w1 = (r[1] >> 1) | (r[2] << 30);
-    const uint32_t w2 = (r[2] >> 2) | (r[3] << 29), w3 = r[3] >> 3;
+    // digits -> 32-bit limbs
+    U128 u;
+    u.v[0] = r[0] | (r[1] << 31);
+    u.v[1] = (r[1] >> 1) | (r[2] << 30);
+    u.v[2] = (r[2] >> 2) | (r[3] << 29);
+    u.v[3] = r[3] >> 3;
+    return u;
+}
+
+// x - y if x >= y else x (128-bit limbs).
+BNTT_HD U128 sub_if_ge(const U128 &x, const U128 &y)
+{
     uint32_t k, z0, z1, z2, z3;
-    z0 = subb(w0, mq.ql.v[0], 0, k);
-    z1 = subb(w1, mq.ql.v[1], k, k);
-    z2 = subb(w2, mq.ql.v[2], k, k);
-    z3 = subb(w3, mq.ql.v[3], k, k);
-    U128 res;
-    res.v[0] = k ? w0 : z0;
-    res.v[1] = k ? w1 : z1;
-    res.v[2] = k ? w2 : z2;
-    res.v[3] = k ? w3 : z3;
-    return res;
+    z0 = subb(x.v[0], y.v[0], 0, k);
+    z1 = subb(x.v[1], y.v[1], k, k);
+    z2 = subb(x.v[2], y.v[2], k, k);
+    z3 = subb(x.v[3], y.v[3], k, k);
+    U128 r;
+    r.v[0] = k ? x.v[0] : z0;
+    r.v[1] = k ? x.v[1] : z1;
+    r.v[2] = k ? x.v[2] : z2;
+    r.v[3] = k ? x.v[3] : z3;
+    return r;
+}
+
+// Exact Montgomery product: canonical for a < 4q, w < q.
+BNTT_HD U128 montmul_r31(const U128 &a, const R31 &w, const MontQ &mq)
+{
+    return sub_if_ge(montmul_r31_lazy(a, w, mq), mq.ql);
+}
+
+// Plain 128-bit add and subtract (no reduction).
+BNTT_HD U128 add128l(const U128 &a, const U128 &b)
+{
+    uint32_t k;
+    U128 r;
+    r.v[0] = addc(a.v[0], b.v[0], 0, k);
+    r.v[1] = addc(a.v[1], b.v[1], k, k);
+    r.v[2] = addc(a.v[2], b.v[2], k, k);
+    r.v[3] = addc(a.v[3], b.v[3], k, k);
+    return r;
+}
+BNTT_HD U128 sub128l(const U128 &a, const U128 &b)
+{
+    uint32_t k;
+    U128 r;
+    r.v[0] = subb(a.v[0], b.v[0], 0, k);
+    r.v[1] = subb(a.v[1], b.v[1], k, k);
+    r.v[2] = subb(a.v[2], b.v[2], k, k);
+    r.v[3] = subb(a.v[3], b.v[3], k, k);
+    return r;
 }
 
 // Canonical (a + b) mod q and (a - b) mod q for a, b < q.

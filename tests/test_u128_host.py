@@ -100,3 +100,37 @@ def test_montgomery_and_canonical_ops(chk):
     assert got == [x * y * rinv % q for x, y in zip(av, bv)]
     assert to_ints(run(chk, 8, a, b, q, 0)) == [(x + y) % q for x, y in zip(av, bv)]
     assert to_ints(run(chk, 9, a, b, q, 0)) == [(x - y) % q for x, y in zip(av, bv)]
+
+
+def test_lazy_montgomery(chk):
+    """Lazy Montgomery product and butterfly (serving path): for inputs below
+    4q and a canonical twiddle the product is below 2q and congruent to
+    a*w*2^-124; the butterfly's outputs are below 4q and congruent to
+    x + y*w*2^-124 and x - y*w*2^-124.  Also the exact product on inputs up to
+    4q.  Moduli: q_116, the largest prime below 2^122 with 2^17 | q - 1, and 97."""
+    from benchntt_amd.residues import to_ints
+
+    R = 1 << 124
+    for q in (C.Q_116, (1 << 122) - 6160383, 97):
+        qinv = (-pow(q, -1, R)) % R
+        rinv = pow(R, -1, q)
+        rng = random.Random(q)
+        edges = [0, 1, q - 1, q, 2 * q - 1, 2 * q, 4 * q - 1, 3 * q + 5]
+        av = [e for e in edges for _ in edges] + [rng.randrange(4 * q) for _ in range(5000)]
+        bv = edges * len(edges) + [rng.randrange(q) for _ in range(5000)]
+        bv = [y % q for y in bv]  # the twiddle operand is canonical
+        a, b = from_ints(av), from_ints(bv)
+        lazy = to_ints(run(chk, 10, a, b, q, qinv))
+        assert all(v < 2 * q and v % q == x * y * rinv % q for v, x, y in zip(lazy, av, bv))
+        assert to_ints(run(chk, 7, a, b, q, qinv)) == [x * y * rinv % q for x, y in zip(av, bv)]
+        # butterfly with a one-digit twiddle w < min(q, 2^31)
+        w = rng.randrange(min(q, 1 << 31))
+        ys = from_ints([rng.randrange(4 * q) for _ in range(len(av))])
+        out = np.zeros((2 * len(av), 2), dtype=np.uint64)
+        mm, mmu = from_ints([q]), from_ints([qinv])
+        chk.chk_op(11, out.ctypes.data, a.ctypes.data, ys.ctypes.data, len(av), mm.ctypes.data, mmu.ctypes.data, w)
+        o = to_ints(out)
+        for i, (x, y) in enumerate(zip(av, to_ints(ys))):
+            p = y * w * rinv
+            assert o[2 * i] < 4 * q and o[2 * i] % q == (x + p) % q
+            assert o[2 * i + 1] < 4 * q and o[2 * i + 1] % q == (x - p) % q
