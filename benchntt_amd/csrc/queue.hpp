@@ -195,28 +195,33 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         } else if (mlog == 0) {
             const int g = e >> k, ti = e & ((1 << k) - 1);
             if (g < nvalid) {
-                U128 a;
                 if (gather) {
-                    const uint32_t r = bitrev((uint32_t)ti, (uint32_t)L);
-                    a = gather_product(ld16(s_x[g] + r), ld16(s_x2[g] + r), md.mq);
+                    // network input brv(j) is A[j] * B[j]: contiguous loads,
+                    // bit-reversed placement in LDS
+                    const U128 a = gather_product(ld16(s_x[g] + ti), ld16(s_x2[g] + ti), md.mq);
+                    lds[lidx((g << k) + (int)bitrev((uint32_t)ti, (uint32_t)k), k)] = a;
                 } else {
-                    a = ld16(s_x[g] + ti);
+                    const U128 a = ld16(s_x[g] + ti);
+                    big |= is_big(a);
+                    lds[lidx(e, k)] = a;
                 }
-                big |= is_big(a);
-                lds[lidx(e, k)] = a;
+            }
+        } else if (gather) {
+            // group g's inputs (ti << mlog) + M, bit-reversed, are the
+            // contiguous block brv(M) << k .. + 2^k of A and B (in the order
+            // brv(ti)): load the block, place each value at its ti
+            const int g = e >> k, j = e & ((1 << k) - 1);
+            if (g < nvalid) {
+                const uint32_t blk = bitrev((uint32_t)it.first + (uint32_t)g, (uint32_t)mlog) << k;
+                const U128 a = gather_product(ld16(d0.x + blk + j), ld16(d0.x2 + blk + j), md.mq);
+                lds[lidx((g << k) + (int)bitrev((uint32_t)j, (uint32_t)k), k)] = a;
             }
         } else {
             const int g = e & (G - 1), ti = e >> Glog;
             if (g < nvalid) {
                 // indices within one transform (at most 2^24 points) fit 32 bits
                 const uint32_t idx = ((uint32_t)ti << mlog) + (uint32_t)it.first + (uint32_t)g;
-                U128 a;
-                if (gather) {
-                    const uint32_t r = bitrev(idx, (uint32_t)L);
-                    a = gather_product(ld16(d0.x + r), ld16(d0.x2 + r), md.mq);
-                } else {
-                    a = ld16(src + idx);
-                }
+                const U128 a = ld16(src + idx);
                 big |= is_big(a);
                 lds[lidx((g << k) + ti, k)] = a;
             }
@@ -285,10 +290,14 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             const int g = e >> k;
             if (g < nvalid) {
                 const uint32_t ti = (uint32_t)(e & ((1 << k) - 1));
-                if (scatter)
-                    st16(s_y[g] + bitrev(ti, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[ti], md.mq));
-                else
+                if (scatter) {
+                    // position ti of c takes network output brv(ti): contiguous
+                    // stores, factor from the table permuted to positions
+                    const U128 y = lds[lidx((g << k) + (int)bitrev(ti, (uint32_t)k), k)];
+                    st16(s_y[g] + ti, montmul_r31(y, d0.scale[((size_t)1 << k) + ti], md.mq));
+                } else {
                     st16(s_y[g] + ti, lds[lidx(e, k)]);
+                }
             }
         } else {
             if ((e >> k) < nvalid) {

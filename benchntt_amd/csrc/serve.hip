@@ -58,7 +58,8 @@ uint32_t brv(uint32_t i, int bits) { return bntt::brv_host(i, bits); }
 struct SizeTables {
     R31 *fwd = nullptr;    // tw_psi * R, digits
     R31 *inv = nullptr;    // tw_inv * R, digits
-    R31 *scale = nullptr;  // N^-1 psi^-brv(i) R^2, digits
+    R31 *scale = nullptr;  // N^-1 psi^-brv(i) R^2 by network output index i, then (entries
+                           // N..2N-1) the same permuted to output position brv(i), digits
 };
 
 // First-fit allocator over [0, size) in 256-byte units.
@@ -203,7 +204,7 @@ int build_tables(bntt_pm_server *s, int L)
     if (!hm.two_adic_root(&w, &v)) return bntt_set_error(BNTT_EINVAL, "no quadratic non-residue below 1000");
     if (L + 1 > v) return bntt_set_error(BNTT_EINVAL, "q - 1 has too few factors of two for this size");
     const size_t n = (size_t)1 << L;
-    std::vector<R31> fwd(n), inv(n), scale(n);
+    std::vector<R31> fwd(n), inv(n), scale(2 * n);
     fwd[0] = bntt::to_r31(hm.r1);  // entry 0 is never read
     inv[0] = fwd[0];
     for (int st = 0; st < L; st++) {
@@ -230,17 +231,20 @@ int build_tables(bntt_pm_server *s, int L)
     U128 acc = ninv;  // Montgomery form of N^-1 psi^-j
     for (size_t j = 0; j < n; j++) {
         // stored = value * R^2 = montmul(value * R, R^2)
-        scale[brv((uint32_t)j, L)] = bntt::to_r31(bntt::montmul_r31(acc, hm.r2, hm.mq));
+        // position j of c takes network output i = brv(j)
+        const R31 sj = bntt::to_r31(bntt::montmul_r31(acc, hm.r2, hm.mq));
+        scale[brv((uint32_t)j, L)] = sj;
+        scale[n + j] = sj;
         acc = hm.mul(acc, psi_inv);
     }
     SizeTables &t = s->tabs[L];
     const size_t bytes = n * sizeof(R31);
     if (hipMalloc(&t.fwd, bytes) != hipSuccess || hipMalloc(&t.inv, bytes) != hipSuccess ||
-        hipMalloc(&t.scale, bytes) != hipSuccess)
+        hipMalloc(&t.scale, 2 * bytes) != hipSuccess)
         return bntt_set_error(BNTT_ENOMEM, "hipMalloc for polynomial-multiply tables failed");
     if (hipMemcpy(t.fwd, fwd.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(t.inv, inv.data(), bytes, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(t.scale, scale.data(), bytes, hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(t.scale, scale.data(), 2 * bytes, hipMemcpyHostToDevice) != hipSuccess)
         return bntt_set_error(BNTT_EHIP, "uploading polynomial-multiply tables failed");
     return BNTT_OK;
 }
