@@ -20,20 +20,15 @@ import numpy as np
 
 
 def valid(L: int, elog: int, passes: int) -> bool:
-    """Same rule as the planner: every pass fits the tile and holds at least
-    elog stages."""
+    """Same rule as the planner (pass_split in csrc/ntt_core.hpp): the stages
+    can be split into `passes` passes of elog..8+elog stages each."""
     tile_log = 8 + elog
     if not (1 <= elog <= 3 and 1 <= passes <= 3):
         return False
     if passes == 1:
         return elog <= L <= tile_log
-    S = 0
-    for p in range(passes):
-        k = (L - S) // (passes - p)
-        if k > tile_log or k < elog:
-            return False
-        S += k
-    return True
+    lo, hi = elog * passes, tile_log * passes
+    return lo <= L <= hi
 
 
 def candidates(L: int):

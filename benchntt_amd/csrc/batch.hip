@@ -117,12 +117,8 @@ bool valid_choice(int L, const TuneChoice &c)
     const int tile_log = 8 + c.elog;
     if (c.np == 1) return L <= tile_log && L >= c.elog;
     // every pass must fit the tile and hold at least elog stages
-    for (int p = 0, S = 0; p < c.np; p++) {
-        const int k = (L - S) / (c.np - p);
-        if (k > tile_log || k < c.elog) return false;
-        S += k;
-    }
-    return true;
+    int ks[3];
+    return bntt::pass_split(L, c.np, c.elog, tile_log, ks);
 }
 
 }  // namespace
@@ -356,9 +352,11 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         } else {
             const uint32_t di = (uint32_t)packed.size();
             packed.push_back(descs[ti]);
+            int ks[3];
+            bntt::pass_split(L, c.np, c.elog, 8 + c.elog, ks);  // valid: checked with the choice
             int S = 0;
             for (int pidx = 0; pidx < c.np; pidx++) {
-                const int k = (L - S) / (c.np - pidx);
+                const int k = ks[pidx];
                 const int Glog = 8 + c.elog - k;
                 const size_t groups = (size_t)1 << (L - k);
                 for (size_t g0 = 0; g0 < groups; g0 += ((size_t)1 << Glog)) {

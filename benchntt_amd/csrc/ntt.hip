@@ -151,9 +151,11 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
     }
     if (tmp == nullptr) return hipErrorInvalidValue;
     const int np = (L + 7) / 8;
+    int ks[3];
+    if (!pass_split(L, np, 3, 11, ks)) return hipErrorInvalidValue;
     int S = 0;
     for (int p = 0; p < np; p++) {
-        const int k = (L - S) / (np - p);
+        const int k = ks[p];
         const int Glog = 11 - k;  // 2048 values per workgroup
         const U128 *src = (p == 0) ? x : ((np - p) % 2 == 0 ? y : tmp);
         U128 *dst = ((np - 1 - p) % 2 == 0) ? y : tmp;

@@ -62,6 +62,43 @@ constexpr size_t kLds3Bytes = 53 * 1024;
 // lanes take consecutive groups).  Staged twiddle entry e sits at tidx(e).
 __device__ __forceinline__ int lidx(int i, int k) { return i + (i >> 3) + (i >> k); }
 __device__ __forceinline__ int tidx(int e) { return e + (e >> 3); }
+// Stages per pass of an np-pass (1..3) transform of 2^L points whose passes
+// hold 2^k-point groups (elog <= k <= tile_log) and run register sub-passes of
+// up to elog stages: the split with the fewest sub-passes (sum of
+// ceil(k / elog)), then the smallest largest pass, then the most even, then
+// the smaller passes first (pass 0 gathers the longest contiguous runs).
+// False if none exists.
+__host__ __device__ inline bool pass_split(int L, int np, int elog, int tile_log, int *ks)
+{
+    if (np == 1) {
+        ks[0] = L;
+        return L >= (elog < L ? elog : L) && L <= tile_log;
+    }
+    int best = 1 << 30;
+    bool found = false;
+    for (int k0 = elog; k0 <= tile_log; k0++) {
+        for (int k1 = elog; k1 <= tile_log; k1++) {
+            const int k2 = L - k0 - k1;
+            if (np == 2 && k2 != 0) continue;
+            if (np == 3 && (k2 < elog || k2 > tile_log)) continue;
+            const int k2e = np == 3 ? k2 : k1;  // np = 2: two passes
+            const int kmax = k0 > k1 ? (k0 > k2e ? k0 : k2e) : (k1 > k2e ? k1 : k2e);
+            const int kmin = k0 < k1 ? (k0 < k2e ? k0 : k2e) : (k1 < k2e ? k1 : k2e);
+            const int subs = (k0 + elog - 1) / elog + (k1 + elog - 1) / elog + (np == 3 ? (k2 + elog - 1) / elog : 0);
+            const bool ascending = k0 <= k1 && (np == 2 || k1 <= k2);
+            const int key = subs * 65536 + kmax * 256 + (kmax - kmin) * 2 + (ascending ? 0 : 1);
+            if (key < best) {
+                best = key;
+                ks[0] = k0;
+                ks[1] = k1;
+                if (np == 3) ks[2] = k2;
+                found = true;
+            }
+        }
+    }
+    return found;
+}
+
 __host__ __device__ inline size_t lds_data_slots(int k, int Glog)
 {
     const size_t n = (size_t)1 << (Glog + k);

@@ -134,12 +134,13 @@ size_t items_per_request(int L, int want)
 {
     const int np = passes_for(L);
     if (np == 1) return 3;
+    const int elog = elog_for(L, want);
+    int ks[3];
+    bntt::pass_split(L, np, elog, 8 + elog, ks);
     size_t per_transform = 0;
-    for (int p = 0, S = 0; p < np; p++) {
-        const int k = (L - S) / (np - p);
-        const int Glog = 8 + elog_for(L, want) - k;
-        per_transform += (((size_t)1 << (L - k)) + ((size_t)1 << Glog) - 1) >> Glog;
-        S += k;
+    for (int p = 0; p < np; p++) {
+        const int Glog = 8 + elog - ks[p];
+        per_transform += (((size_t)1 << (L - ks[p])) + ((size_t)1 << Glog) - 1) >> Glog;
     }
     return 3 * per_transform;
 }
@@ -296,11 +297,13 @@ void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector
         }
         return;
     }
+    int ks[3];
+    bntt::pass_split(L, np, elog, 8 + elog, ks);
     for (const DevDesc &d : group) {
         const uint32_t di = (uint32_t)descs.size();
         descs.push_back(d);
         for (int p = 0, S = 0; p < np; p++) {
-            const int k = (L - S) / (np - p);
+            const int k = ks[p];
             const int Glog = 8 + elog - k;
             const size_t groups = (size_t)1 << (L - k);
             for (size_t g0 = 0; g0 < groups; g0 += ((size_t)1 << Glog)) {
