@@ -7,15 +7,17 @@
 //   - a single-pass item runs all stages of up to 2048/N whole transforms
 //     (N <= 2048) that share one twiddle table;
 //   - a multi-pass transform (larger N) contributes one item per 2048-value
-//     block to each of its ceil(log2 N / 8) passes, pass p at level p, through
-//     a plan-owned intermediate buffer;
+//     block to each of its passes (ceil(log2 N / 8) by default; stages split
+//     by pass_split), pass p at level p, through a plan-owned intermediate
+//     buffer;
 //   - a product contributes one item per 2048 elements at the level after the
 //     last pass of the transforms whose outputs it reads.
-// Each level is one launch of the same persistent kernel: a grid sized to the
-// resident capacity of the chip whose workgroups take items from an atomic
-// ticket until the level's list is exhausted, the last workgroup resetting the
-// counters for the next launch.  Items are ordered largest-first with the
-// memory-bound products spread among the compute-bound transform items.
+// Each level is one launch of the same persistent kernel (queue.hpp): a grid
+// sized to the resident capacity of the chip whose workgroups start with the
+// item of their own index and draw the rest from an atomic counter until the
+// level's list is exhausted, the last draw rearming the counter for the next
+// launch.  Items are ordered largest-first with the memory-bound products
+// spread among the compute-bound transform items.
 // Kernel boundaries order the levels, so no workgroup ever waits on another.
 //
 // The radix (points per thread per register sub-pass: 2, 4 or 8, hence the

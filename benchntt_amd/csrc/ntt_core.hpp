@@ -1,6 +1,7 @@
 // ntt_core.hpp -- device building blocks of the forward constant-geometry NTT,
 // shared by the launch-per-call kernel (ntt.hip) and the persistent
-// work-queue kernel (batch.hip).
+// work-queue kernel (queue.hpp: uneven batches in batch.hip, polynomial
+// products in serve.hip).
 //
 // Reference network (ntt_scalar.c:582-627): log2(N) stages; at stage s
 // (t = 2^s), for j in [0, N/2):

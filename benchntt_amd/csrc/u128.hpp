@@ -20,8 +20,17 @@
 //                single conditional subtract returns exactly a*b mod q for all
 //                a, b < 2^124; mulmod_pm returns the same value with about half
 //                the multiplies (two folds of 2^124 = c mod q).
+//   mulmod_r31   the same exact product with a in 31-bit digits (the second
+//                operand pre-converted: twiddles, the axpy scalar): 16
+//                multiply-adds with no carry fix-ups; the fast path of the
+//                network and of vvmul/axpy.
 //   mulmod       mulmod_pm when the modulus qualifies and both operands are
 //                below 2^124, mulmod_ref otherwise: bit-exact in every case.
+// and, for the polynomial-multiply path (any odd q < 2^122, exact results):
+//   montmul_r31_lazy / montmul_r31   Montgomery product a*w*2^-124, below 2q
+//                / canonical, for a < 4q and w < q
+//   addmod_canon / submod_canon      canonical add/sub; add128l / sub128l /
+//                sub_if_ge for lazy butterflies (values below 4q).
 #pragma once
 
 #include <hip/hip_runtime.h>
