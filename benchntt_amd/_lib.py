@@ -117,6 +117,7 @@ _SIGS = {
     "bntt_pm_server_create": [ctypes.POINTER(ctypes.c_void_p), _PU, ctypes.POINTER(PmConfig)],
     "bntt_pm_server_destroy": [_P],
     "bntt_pm_multiply": [_P, ctypes.POINTER(PmRequest), _SZ],
+    "bntt_pm_transform": [_P, ctypes.c_int, _P, _P, ctypes.c_int, _SZ],
     "bntt_pm_run_trace": [_P, ctypes.POINTER(PmRequest), ctypes.POINTER(ctypes.c_uint64), _SZ,
                           ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(PmStats)],
     "bntt_hilo_to_u128": [_P, _P, _SZ],
@@ -139,6 +140,7 @@ _OTHER = {
 }
 
 BNTT_TW_CYCLIC, BNTT_TW_NEGACYCLIC, BNTT_TW_CYCLIC_INV, BNTT_TW_NEGACYCLIC_INV = 0, 1, 2, 3
+BNTT_PM_FORWARD, BNTT_PM_INVERSE = 0, 1
 
 EXPORTS = tuple(_SIGS) + tuple(_OTHER)
 

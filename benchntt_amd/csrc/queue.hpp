@@ -12,7 +12,9 @@
 // With Montgomery arithmetic a transform item may also fuse the pointwise
 // steps of a negacyclic product into its first load and its last store:
 //   IT_GATHER_MUL     the first pass reads x[i] = A[brv(i)] * B[brv(i)] * R^-1
+//   IT_GATHER         the first pass reads x[i] = A[brv(i)] * R^-1
 //   IT_SCATTER_SCALE  the last pass writes C[brv(i)] = y[i] * S[i] * R^-1
+//   IT_CANON          the last pass writes y canonical (values in [0, q))
 // (values between the Montgomery steps are kept below 4q; C is canonical)
 // where brv reverses the log2 N index bits of a transform.  IT_CONTIG marks a
 // packed item whose transforms lie back to back in x and in y: it is read and
@@ -33,7 +35,7 @@ constexpr size_t kQueueMaxLds = 80 * 1024;
 constexpr size_t kQueueLds3 = 46 * 1024;
 
 enum ItemKind : uint8_t { ITEM_NTT = 0, ITEM_MUL = 1 };
-enum ItemFlags : uint8_t { IT_GATHER_MUL = 1, IT_SCATTER_SCALE = 2, IT_CONTIG = 4 };
+enum ItemFlags : uint8_t { IT_GATHER_MUL = 1, IT_SCATTER_SCALE = 2, IT_CONTIG = 4, IT_GATHER = 8, IT_CANON = 16 };
 
 struct DevDesc {
     const U128 *x;
@@ -135,12 +137,17 @@ inline size_t plan_tw_area(size_t data_slots, const size_t *tw_needs, size_t n, 
     return data + sizeof(R31) * cap;
 }
 
+// a value below 4q brought into [0, q)
+__device__ __forceinline__ U128 canonical(const U128 &a, const MontQ &mq)
+{
+    return sub_if_ge(sub_if_ge(a, mq.ql2), mq.ql);
+}
+
 // A * B * R^-1 for forward outputs A, B below 4q (lazy butterflies): B is
 // brought below q, the product is below 2q (a valid network input).
 __device__ __forceinline__ U128 gather_product(const U128 &a, const U128 &b, const MontQ &mq)
 {
-    const U128 bq = sub_if_ge(sub_if_ge(b, mq.ql2), mq.ql);
-    return montmul_r31_lazy(a, to_r31(bq), mq);
+    return montmul_r31_lazy(a, to_r31(canonical(b, mq)), mq);
 }
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
@@ -169,9 +176,13 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         dst = ((np - 1 - p) % 2 == 0) ? d0.y : d0.tmp;
         src = (p == 0) ? d0.x : (((np - p) % 2 == 0) ? d0.y : d0.tmp);
     }
-    // fused pointwise steps (Montgomery products only)
-    const bool gather = MONT && (it.flags & IT_GATHER_MUL) && it.pass == 0;
+    // fused pointwise steps (Montgomery products only); a plain gather is a
+    // product with B = 1
+    const bool gather = MONT && (it.flags & (IT_GATHER_MUL | IT_GATHER)) && it.pass == 0;
+    const bool gmul = (it.flags & IT_GATHER_MUL) != 0;
+    const U128 one{{1u, 0u, 0u, 0u}};
     const bool scatter = MONT && (it.flags & IT_SCATTER_SCALE) && it.pass + 1 == it.np;
+    const bool canon = MONT && (it.flags & IT_CANON) && it.pass + 1 == it.np;
     const bool contig = !MONT && (it.flags & IT_CONTIG);
     if (mlog == 0 && !contig) {
         // whole transforms, one per group, each with its own buffers
@@ -198,7 +209,7 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                 if (gather) {
                     // network input brv(j) is A[j] * B[j]: contiguous loads,
                     // bit-reversed placement in LDS
-                    const U128 a = gather_product(ld16(s_x[g] + ti), ld16(s_x2[g] + ti), md.mq);
+                    const U128 a = gather_product(ld16(s_x[g] + ti), gmul ? ld16(s_x2[g] + ti) : one, md.mq);
                     lds[lidx((g << k) + (int)bitrev((uint32_t)ti, (uint32_t)k), k)] = a;
                 } else {
                     const U128 a = ld16(s_x[g] + ti);
@@ -213,7 +224,7 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             const int g = e >> k, j = e & ((1 << k) - 1);
             if (g < nvalid) {
                 const uint32_t blk = bitrev((uint32_t)it.first + (uint32_t)g, (uint32_t)mlog) << k;
-                const U128 a = gather_product(ld16(d0.x + blk + j), ld16(d0.x2 + blk + j), md.mq);
+                const U128 a = gather_product(ld16(d0.x + blk + j), gmul ? ld16(d0.x2 + blk + j) : one, md.mq);
                 lds[lidx((g << k) + (int)bitrev((uint32_t)j, (uint32_t)k), k)] = a;
             }
         } else {
@@ -296,7 +307,8 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                     const U128 y = lds[lidx((g << k) + (int)bitrev(ti, (uint32_t)k), k)];
                     st16(s_y[g] + ti, montmul_r31(y, d0.scale[((size_t)1 << k) + ti], md.mq));
                 } else {
-                    st16(s_y[g] + ti, lds[lidx(e, k)]);
+                    const U128 y = lds[lidx(e, k)];
+                    st16(s_y[g] + ti, canon ? canonical(y, md.mq) : y);
                 }
             }
         } else {
@@ -304,6 +316,8 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                 const uint32_t idx = ((uint32_t)it.first << k) + (uint32_t)e;
                 if (scatter)
                     st16(dst + bitrev(idx, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[idx], md.mq));
+                else if (canon)
+                    st16(dst + idx, canonical(lds[lidx(e, k)], md.mq));
                 else
                     st16(dst + idx, lds[lidx(e, k)]);
             }

@@ -327,7 +327,18 @@ void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector
     }
 }
 
-// Build the four levels of a batch into the slot's staging buffer and launch them.
+int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<WorkItem>> &lv,
+                  const std::vector<DevDesc> &descs);
+
+// 2^want points per thread for a batch of items8 radix-8 items: a batch whose
+// radix-8 items would leave resident workgroups idle runs 512-value items,
+// one that would fill fewer than three rounds 1024-value items
+int want_for(const bntt_pm_server *s, size_t items8)
+{
+    return items8 < (size_t)s->grid_cap ? 1 : items8 < 3 * (size_t)s->grid_cap ? 2 : 3;
+}
+
+// Build the four levels of a batch of requests and launch them.
 int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
 {
     // levels: 0, 1 forward passes; 2, 3 inverse passes (single-pass sizes use 0 and 2)
@@ -335,12 +346,9 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
     std::vector<std::vector<WorkItem>> lv(kLevels);
     std::vector<DevDesc> descs;
     U128 *arena = reinterpret_cast<U128 *>(s->arena_d);
-    // a batch whose radix-8 items would leave resident workgroups idle runs
-    // 1024-value items (each level finishes in about half the time), a
-    // batch that would not fill a quarter of them 512-value items
     size_t items8 = 0;
     for (const ReqView &r : reqs) items8 += items_per_request(r.L, 3);
-    const int want = items8 < (size_t)s->grid_cap ? 1 : items8 < 3 * (size_t)s->grid_cap ? 2 : 3;
+    const int want = want_for(s, items8);
     for (int L = s->min_l; L <= s->max_l; L++) {
         std::vector<DevDesc> fwd, inv;
         for (const ReqView &r : reqs) {
@@ -363,6 +371,16 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
             add_transforms(lv, 2, descs, inv, L, bntt::IT_GATHER_MUL | bntt::IT_SCATTER_SCALE, want);
         }
     }
+    return launch_levels(s, sl, lv, descs);
+}
+
+// Copy a batch's levels into the slot's staging buffer, upload them, launch
+// one work-queue kernel per non-empty level on the slot's stream (each level
+// starts when the previous one ends) and record the slot's event.
+int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<WorkItem>> &lv,
+                  const std::vector<DevDesc> &descs)
+{
+    const int kLevels = (int)lv.size();
     size_t nitems = 0;
     for (auto &v : lv) nitems += v.size();
     if (nitems > sl.cap_items || descs.size() > sl.cap_descs)
@@ -540,6 +558,66 @@ int bntt_pm_multiply(bntt_pm_server *s, const bntt_pm_request *reqs, size_t n)
         if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = bntt_set_error(BNTT_EHIP, "batch failed");
     }
     for (auto &a : allocs) s->arena.release(a.first, a.second);
+    return rc;
+}
+
+int bntt_pm_transform(bntt_pm_server *s, int direction, void *y, const void *x, int log2n, size_t count)
+{
+    if (!s) return bntt_set_error(BNTT_EINVAL, "null argument");
+    if (direction != BNTT_PM_FORWARD && direction != BNTT_PM_INVERSE)
+        return bntt_set_error(BNTT_EINVAL, "direction must be BNTT_PM_FORWARD or BNTT_PM_INVERSE");
+    if (log2n < s->min_l || log2n > s->max_l) return bntt_set_error(BNTT_EINVAL, "transform size not served");
+    if (count == 0) return BNTT_OK;
+    if (!x || !y || ((reinterpret_cast<uintptr_t>(x) | reinterpret_cast<uintptr_t>(y)) & 15u))
+        return bntt_set_error(BNTT_EINVAL, "pointers must be non-null and 16-byte aligned");
+    const int L = log2n;
+    const size_t n = (size_t)1 << L;
+    if (count > (SIZE_MAX / sizeof(U128)) >> L) return bntt_set_error(BNTT_EINVAL, "count too large");
+    const uintptr_t xa = reinterpret_cast<uintptr_t>(x), ya = reinterpret_cast<uintptr_t>(y);
+    const uintptr_t bytes = (uintptr_t)(count << L) * sizeof(U128);
+    if (xa != ya && xa < ya + bytes && ya < xa + bytes)
+        return bntt_set_error(BNTT_EINVAL, "y must be x or disjoint from it");
+    // transforms per launch: within the slot's descriptors and, for two-pass
+    // sizes, one pass buffer each within the HBM budget (every arena block is
+    // free between calls)
+    size_t chunk = 3 * (size_t)s->max_batch;
+    size_t per = 0, off = 0;
+    if (passes_for(L) > 1) {
+        per = (n * sizeof(U128) + kUnit - 1) / kUnit;
+        chunk = std::min(chunk, s->arena.total() / per);
+        if (chunk == 0) return bntt_set_error(BNTT_EINVAL, "a transform needs more than the whole HBM budget");
+        chunk = std::min(chunk, count);
+        if (!s->arena.alloc(chunk * per, &off)) return bntt_set_error(BNTT_ENOMEM, "serving arena in use");
+    }
+    const bool fwd = direction == BNTT_PM_FORWARD;
+    const uint8_t flags = fwd ? bntt::IT_CANON : (bntt::IT_GATHER | bntt::IT_SCATTER_SCALE);
+    const SizeTables &tab = s->tabs[L];
+    U128 *tmp = reinterpret_cast<U128 *>(s->arena_d) + off * (kUnit / sizeof(U128));
+    Slot &sl = s->slots.back();
+    int rc = BNTT_OK;
+    for (size_t t0 = 0; t0 < count && !rc; t0 += chunk) {
+        const size_t m = std::min(chunk, count - t0);
+        std::vector<DevDesc> group(m);
+        for (size_t t = 0; t < m; t++) {
+            DevDesc &d = group[t];
+            d.x = static_cast<const U128 *>(x) + ((t0 + t) << L);
+            d.y = static_cast<U128 *>(y) + ((t0 + t) << L);
+            d.tmp = per ? tmp + t * per * (kUnit / sizeof(U128)) : nullptr;
+            d.twr = fwd ? tab.fwd : tab.inv;
+            d.scale = fwd ? nullptr : tab.scale;
+        }
+        std::vector<std::vector<WorkItem>> lv(2);
+        std::vector<DevDesc> descs;
+        // radix-8 items of the chunk: whole transforms packed 2^Glog per item
+        // for one-pass sizes, else blocks of one transform
+        const int glog8 = 8 + elog_for(L, 3) - L;
+        const size_t items8 = passes_for(L) == 1 ? (m + ((size_t)1 << glog8) - 1) >> glog8
+                                                 : m * (items_per_request(L, 3) / 3);
+        add_transforms(lv, 0, descs, group, L, flags, want_for(s, items8));
+        rc = launch_levels(s, sl, lv, descs);
+        if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = bntt_set_error(BNTT_EHIP, "transform failed");
+    }
+    if (per) s->arena.release(off, chunk * per);
     return rc;
 }
 

@@ -4,9 +4,12 @@ bntt_pm_* in include/bntt.h).
     srv = PolymulServer(q=config.Q_116, hbm_budget=1 << 30)
     srv.multiply([(a, b, c), ...])              # one batch, returns when done
     done_ns, stats = srv.run_trace(reqs, arrival_ns)
+    srv.forward(y, x, n); srv.inverse(x, y, n)  # the pipeline's two transforms
 
 a, b, c are device tensors of N canonical residues (torch int64 (N, 2)); the
-product c = a * b mod (x^N + 1, q).  No counterpart in the reference.
+product c = a * b mod (x^N + 1, q).  forward / inverse take count transforms of
+n points back to back: the negacyclic NTT with outputs in bit-reversed order,
+and its inverse.  No counterpart in the reference.
 """
 from __future__ import annotations
 
@@ -53,6 +56,24 @@ class PolymulServer:
         for i in range(0, len(reqs), self.max_batch):
             chunk = reqs[i:i + self.max_batch]
             _lib.call("bntt_pm_multiply", self._h, _requests(chunk), len(chunk))
+
+    def _transform(self, direction, y, x, n):
+        total = _nelem(x)
+        l = n.bit_length() - 1
+        if n != 1 << l or total % n or _nelem(y) != total:
+            raise ValueError("x and y must hold the same whole number of n-point transforms, n a power of two")
+        _ready()
+        _lib.call("bntt_pm_transform", self._h, direction, _dev_ptr(y), _dev_ptr(x), l, total // n)
+
+    def forward(self, y, x, n: int) -> None:
+        """y = the negacyclic NTT of each n-point block of x, outputs of a
+        block in bit-reversed order (y may be x)."""
+        self._transform(_lib.BNTT_PM_FORWARD, y, x, n)
+
+    def inverse(self, y, x, n: int) -> None:
+        """The inverse of forward: x in forward's output order, y the
+        coefficients (y may be x)."""
+        self._transform(_lib.BNTT_PM_INVERSE, y, x, n)
 
     def run_trace(self, reqs, arrival_ns):
         reqs = list(reqs)

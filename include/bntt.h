@@ -214,6 +214,21 @@ int bntt_pm_server_create(bntt_pm_server **server, const bntt_u128 *q, const bnt
 int bntt_pm_server_destroy(bntt_pm_server *server);
 /* Multiply n <= max_batch requests as one batch; returns when c is written. */
 int bntt_pm_multiply(bntt_pm_server *server, const bntt_pm_request *reqs, size_t n);
+
+/* The two halves of the product pipeline on their own, for count transforms of
+ * N = 2^log2n residues lying back to back; returns when y is written.  y may
+ * equal x (in place) but may not overlap it otherwise.  Two-pass sizes (log2n
+ * > 11) take N*16 bytes of the HBM budget per transform of a launch.
+ *   BNTT_PM_FORWARD  y[i] = sum_j x[j] psi^((2 brv(i) + 1) j) for the server's
+ *                    root psi of order 2N (the negacyclic NTT, outputs in
+ *                    bit-reversed order brv of the log2n index bits): the
+ *                    forward network with the table psi_s^(2 brv_s(k) + 1).
+ *   BNTT_PM_INVERSE  its inverse: x in BNTT_PM_FORWARD's output order, y the
+ *                    coefficients in natural order.
+ * The pointwise product of two forward outputs, taken mod q, is the forward
+ * output of the negacyclic product. */
+enum { BNTT_PM_FORWARD = 0, BNTT_PM_INVERSE = 1 };
+int bntt_pm_transform(bntt_pm_server *server, int direction, void *y, const void *x, int log2n, size_t count);
 /* Serve requests that arrive at arrival_ns[i] (sorted, ns after the call
  * starts): arrived requests are admitted in order while the HBM budget has
  * room, batched, and launched on a free stream while earlier batches still
