@@ -70,6 +70,8 @@ class BatchPlan:
 
     def __init__(self, tasks, muls=(), modulus: int = 0, mu: int = 0, tune=None, graph: bool = True,
                  tw_mode: str = "global"):
+        if tw_mode not in TW_FLAGS:
+            raise ValueError(f"tw_mode must be one of {sorted(TW_FLAGS)}")
         self._keep = (list(tasks), list(muls))  # the buffers must outlive the plan
         tarr = _task_array(self._keep[0])
         marr = (MulTask * max(1, len(self._keep[1])))()

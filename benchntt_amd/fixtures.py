@@ -100,6 +100,8 @@ def u128_to_hilo(a, out=None, stream=None):
 
 def _convert(name, src, out, stream):
     n = _nelem(src)
+    if out is not None and _nelem(out) != n:
+        raise ValueError("out must hold as many residues as the input")
     if _is_torch(src):
         if out is None:
             import torch
@@ -119,6 +121,8 @@ def twiddles(log2n: int, q: int, kind: str = "negacyclic", device: bool = True, 
     """Root-of-unity table of 2^log2n entries (kinds: cyclic, negacyclic,
     cyclic_inv, negacyclic_inv), for a prime q < 2^122 with enough factors of
     two in q - 1."""
+    if kind not in KINDS:
+        raise ValueError(f"kind must be one of {sorted(KINDS)}")
     k = KINDS[kind]
     qq = U128.of(q)
     n = 1 << log2n

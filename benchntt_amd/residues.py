@@ -15,7 +15,8 @@ def uniform(n: int, q: int, seed: int) -> np.ndarray:
     """n residues uniform in [0, q) from numpy's PCG64 stream `seed`:
     uniform draws below 2^bitlen(q), with the rare draws >= q redrawn."""
     bits = q.bit_length()
-    assert 65 <= bits <= 128
+    if not 65 <= bits <= 128:
+        raise ValueError("q must have 65 to 128 bits")
     hbits = bits - 64
     rng = np.random.Generator(np.random.PCG64(seed))
     out = np.empty((n, 2), dtype=np.uint64)

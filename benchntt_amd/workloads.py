@@ -26,7 +26,8 @@ class UnevenBatch:
 
 
 def uneven_pairs(count: int, q: int, seed: int, lo: int = 4, hi: int = 16, with_data: bool = True) -> UnevenBatch:
-    assert count % 2 == 0
+    if count % 2:
+        raise ValueError("transforms come in pairs: count must be even")
     rng = np.random.Generator(np.random.PCG64(seed))
     pair_l = rng.integers(lo, hi + 1, size=count // 2)
     log2n = np.repeat(pair_l, 2)

@@ -120,6 +120,11 @@ def test_hilo_host_converters():
         hl = fx.u128_to_hilo(a)
         assert np.array_equal(hl, a[:, ::-1])
         assert np.array_equal(fx.hilo_to_u128(hl), a)
+    out = np.empty((999, 2), dtype=np.uint64)
+    with pytest.raises(ValueError):
+        fx.u128_to_hilo(a, out=out)  # shorter than the input: refused before any write
+    with pytest.raises(ValueError):
+        fx.twiddles(4, C.Q_116, "cyclic-inverse", device=False)
 
 
 def test_text_format_roundtrip(tmp_path, fixtures):
