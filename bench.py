@@ -462,6 +462,16 @@ def main() -> int:
             ms = time_kernel(torch, fn, args.steps, args.warmup, stream)
             extras[f"{name}_GBps"] = round(n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9, 1)
             log(f"{name} 2^24: {ms * 1e3:.1f} us/call, {extras[f'{name}_GBps']} GB/s")
+        # what the device sustains for a plain copy (the runtime's own copy
+        # kernel, 256 MiB read + 256 MiB written): the practical HBM ceiling
+        src = torch.empty(1 << 28, dtype=torch.uint8, device=dev)
+        src.fill_(1)
+        dst = torch.empty_like(src)
+        msc = time_kernel(torch, lambda: dst.copy_(src), args.steps, args.warmup, stream)
+        extras["hbm_device_copy_GBps"] = round(2 * (1 << 28) / (msc * 1e-3) / 1e9, 1)
+        extras["axpy_vs_device_copy"] = round(gbs / extras["hbm_device_copy_GBps"], 4)
+        log(f"device copy 256 MiB: {extras['hbm_device_copy_GBps']} GB/s")
+        del src, dst
 
         cpu = not args.no_cpu_baseline
         extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
