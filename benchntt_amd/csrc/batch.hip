@@ -9,7 +9,8 @@
 //   - a multi-pass transform (larger N) contributes one item per 2048-value
 //     block to each of its passes (ceil(log2 N / 8) by default; stages split
 //     by pass_split), pass p at level p, through a plan-owned intermediate
-//     buffer;
+//     buffer (when the plan has products, the smaller part of the multi-pass
+//     work starts one level later, so that products overlap transforms);
 //   - a product contributes one item per 2048 elements at the level after the
 //     last pass of the transforms whose outputs it reads.
 // Each level is one launch of the same persistent kernel (queue.hpp): a grid
@@ -37,6 +38,13 @@
 #include "../../include/bntt.h"
 #include "internal.hpp"
 #include "ntt_core.hpp"
+
+// share of the multi-pass transform work that starts at level 0 when the
+// plan has products (the rest starts at level 1)
+#ifndef BNTT_STAGGER_SHARE
+#define BNTT_STAGGER_SHARE 0.7
+#endif
+static constexpr double kStaggerShare = BNTT_STAGGER_SHARE;
 
 #include "queue.hpp"
 #include "spans.hpp"
@@ -316,6 +324,23 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         return reinterpret_cast<uintptr_t>(tasks[a].tw) < reinterpret_cast<uintptr_t>(tasks[b].tw);
     });
     std::vector<int> out_level(ntasks, 0);  // level at which each transform's output is final
+    // With products to run, multi-pass transforms start in two groups: the
+    // first kStaggerShare of their work (largest first) at level 0, the rest
+    // one level later.  The products of the first group then run beside the
+    // last pass of the second (memory-bound items among compute-bound ones)
+    // instead of all waiting for a level of their own after every transform.
+    std::vector<int> start_level(ntasks, 0);
+    if (nmuls > 0) {
+        auto work = [&](size_t j) { return (double)tasks[j].log2n * (double)((size_t)1 << tasks[j].log2n); };
+        double total = 0, acc = 0;
+        for (size_t j : order)
+            if (choice[tasks[j].log2n].np > 1) total += work(j);
+        for (size_t j : order)
+            if (choice[tasks[j].log2n].np > 1) {
+                if (acc >= kStaggerShare * total) start_level[j] = 1;
+                acc += work(j);
+            }
+    }
     size_t i0 = 0;
     while (i0 < ntasks) {
         const size_t ti = order[i0];
@@ -374,11 +399,11 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
                     w.desc = di;
                     w.nvalid = (uint32_t)std::min(groups - g0, (size_t)1 << Glog);
                     w.first = g0;
-                    ntt_lv[pidx].push_back(w);
+                    ntt_lv[start_level[ti] + pidx].push_back(w);
                 }
                 S += k;
             }
-            out_level[ti] = c.np - 1;
+            out_level[ti] = start_level[ti] + c.np - 1;
             i0++;
         }
     }

@@ -67,7 +67,13 @@ def test_small_uneven_batch(gpu, orc, tw_mode):
     gpu.cuda.synchronize()
     check_all(orc, wl, from_dev(dy), from_dev(dc))
     info = plan.info()
-    assert info["levels"] == 3 and info["items"] > 0
+    # two-pass transforms in two start groups: passes at levels 0-1 and 1-2,
+    # products after each group's last pass (levels 2 and 3)
+    assert info["levels"] == 4 and info["items"] > 0
+    plan.close()
+    # without products every two-pass transform starts at level 0
+    plan = BatchPlan(tasks, [], QN, MUN, graph=False, tw_mode=tw_mode)
+    assert plan.info()["levels"] == 2
     plan.close()
 
 
