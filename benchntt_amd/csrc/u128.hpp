@@ -122,13 +122,12 @@ BNTT_HD U128 addmod_ref(const U128 &a, const U128 &b, const U128 &m, const U128 
     uint32_t bb, d0, d1, d2, d3;
     d0 = subb(s0, m.v[0], 0, bb);
     d1 = subb(s1, m.v[1], bb, bb);
-    // d6 = t19 - mh - !c5 = t19 + ~mh + c5 (mod 2^64)
-    d2 = addc(s2, notm.v[2], c5, bb);
-    d3 = addc(s3, notm.v[3], bb, bb);
-    // i14 = borrow of (mh - t19 - c5) over 128 bits, i.e. t19 + c5 > mh
+    // d6 = t19 - mh - !c5 = t19 + ~mh + c5 (mod 2^64); the carry out of this
+    // sum is i14, the borrow of (mh - t19 - c5) over 128 bits: t19 + ~mh + c5
+    // reaches 2^64 exactly when t19 + c5 > mh
     uint32_t i14;
-    (void)subb(m.v[2], s2, c5, i14);
-    (void)subb(m.v[3], s3, i14, i14);
+    d2 = addc(s2, notm.v[2], c5, bb);
+    d3 = addc(s3, notm.v[3], bb, i14);
     U128 r;
     r.v[0] = i14 ? d0 : s0;
     r.v[1] = i14 ? d1 : s1;
