@@ -102,10 +102,11 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 8.0):
 
 
 # VALU instructions per butterfly of the radix-8 register network of the NTT
-# kernels (reg_network_fast<3>: 12 butterflies in 1325 VALU instructions,
-# counted in the gfx950 ISA of a kernel that loads 8 values and 7 twiddles,
-# runs the network and stores; the network's product is 20 v_mad_u64_u32)
-VALU_PER_BUTTERFLY = 110
+# kernels (reg_network_fast<3>: 12 butterflies in 1197 VALU instructions on
+# the path every wave takes unless a lane's product may reach q, counted in
+# the gfx950 ISA of a kernel that loads 8 values and 7 twiddles, runs the
+# network and stores; the network's product is 20 v_mad_u64_u32)
+VALU_PER_BUTTERFLY = 100
 
 
 def valu_rates() -> dict:

@@ -104,6 +104,17 @@ BNTT_HD uint32_t funnel(uint32_t hi, uint32_t lo, uint32_t s)
 
 BNTT_HD uint64_t mad32(uint32_t a, uint32_t b, uint64_t c) { return (uint64_t)a * b + c; }
 
+// The predicate holds in some active lane of the wave (on the host: in the one
+// lane there is).  Lets a wave skip a rarely needed exact step as a whole.
+BNTT_HD bool any_lane(bool p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __any(p) != 0;
+#else
+    return p;
+#endif
+}
+
 // ---------------------------------------------------------------- literal add/sub
 
 // ntt_scalar.c:49-79, literal.  notm = ~m.
@@ -302,6 +313,18 @@ BNTT_HD U128 mulmod_r31(const U128 &a_, const R31 &w, uint32_t c)
     w1 = addc((f[1] >> 1) | (f[2] << 30), 0, k, k);
     w2 = addc((f[2] >> 2) | (f[3] << 29), 0, k, k);
     w3 = (f[3] >> 3) + k;  // <= 2^28
+    // w < 2^124 + 2^20 is congruent to a*w; it reaches q = 2^124 - c only with
+    // a top limb of 0x0FFFFFFF or more (about one product in 2^28 on uniform
+    // operands), so a wave skips the exact correction below unless one of its
+    // lanes may need it, and then every lane takes it
+    if (!any_lane(w3 >= 0x0FFFFFFFu)) {
+        U128 r;
+        r.v[0] = w0;
+        r.v[1] = w1;
+        r.v[2] = w2;
+        r.v[3] = w3;
+        return r;
+    }
     uint32_t z0, z1, z2, z3;
     z0 = addc(w0, c, 0, k);
     z1 = addc(w1, 0, k, k);

@@ -75,6 +75,29 @@ def test_edge_operands(gpu, orc, op):
     assert np.array_equal(run_dev(gpu, op, a, b), want(orc, op, a, b))
 
 
+@pytest.mark.parametrize("q,mu", [(C.Q_BLAS, C.MU_BLAS), (C.Q_NTT, C.MU_NTT)], ids=["q_blas", "q_ntt"])
+def test_products_near_zero(gpu, orc, q, mu):
+    """Three lanes in four hold pairs whose product is k < 2^20 mod q (b =
+    a^-1 k; for axpy b = s^-1 k): the fast product's folded value is then k
+    or k + q, so its rarely needed final subtraction runs, in waves mixed
+    with ordinary lanes."""
+    import random
+
+    from benchntt_amd.residues import from_ints
+
+    rng = random.Random(3)
+    n = 16384
+    av, ks = [rng.randrange(1, q) for _ in range(n)], []
+    for i in range(n):
+        ks.append(rng.randrange(1 << 20) if i % 4 else rng.randrange(q))
+    a = from_ints(av)
+    b = from_ints([pow(x, -1, q) * k % q for x, k in zip(av, ks)])
+    assert np.array_equal(run_dev(gpu, "vvmul", a, b, q=q, mu=mu), want(orc, "vvmul", a, b, q=q, mu=mu))
+    s = S % q
+    b2 = from_ints([pow(s, -1, q) * k % q for k in ks])
+    assert np.array_equal(run_dev(gpu, "axpy", a, b2, x=s, q=q, mu=mu), want(orc, "axpy", a, b2, x=s, q=q, mu=mu))
+
+
 @pytest.mark.parametrize("op", ["vvadd", "vvsub", "vvmul", "axpy"])
 def test_other_modulus_literal_path(gpu, orc, op):
     """A modulus outside the fast form (q_116 with the reference's unused

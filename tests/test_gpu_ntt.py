@@ -84,6 +84,24 @@ def test_edge_inputs(gpu, orc):
     assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
 
 
+@pytest.mark.parametrize("L", [10, 14])
+def test_products_near_zero(gpu, orc, L):
+    """The first stage multiplies x[j + N/2] by tw[1]: with x[j + N/2] =
+    tw[1]^-1 k (k < 2^20) in three lanes of four, those products are k mod q,
+    which takes the fast product's rarely needed final subtraction."""
+    import random
+
+    from benchntt_amd.residues import from_ints, to_ints
+
+    n, rng = 1 << L, random.Random(L)
+    x, tw = uniform(2 * n, QN, 60 + L), uniform(n, QN, 61 + L)
+    inv = pow(to_ints(tw[1:2])[0], -1, QN)
+    for t in range(2):
+        hi = [inv * rng.randrange(1 << 20) % QN if j % 4 else rng.randrange(QN) for j in range(n // 2)]
+        x[t * n + n // 2:(t + 1) * n] = from_ints(hi)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
 def test_other_modulus_literal_path(gpu, orc):
     q = C.Q_116
     mu = (1 << 235) // q
