@@ -51,10 +51,13 @@ static constexpr double kStaggerShare = BNTT_STAGGER_SHARE;
 
 namespace bntt {
 
-__global__ __launch_bounds__(256) void tw_flags_clear_kernel(uint32_t *flags, uint32_t n)
+// Clears the table flags and the punt counts at the start of an execution (a
+// kernel rather than memsets, so that graph replays clear them too).
+__global__ __launch_bounds__(256) void exec_clear_kernel(uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t nc)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) flags[i] = 0;
+    if (i < nc) counts[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void tw_convert_kernel(const TwChunk *__restrict__ chunks,
@@ -93,7 +96,9 @@ struct bntt_batch_plan {
     WorkItem *items_d = nullptr;
     DevDesc *descs_d = nullptr;
     DevMul *muls_d = nullptr;
-    bntt::QueueCtl *ctl_d = nullptr;
+    bntt::QueueCtl *ctl_d = nullptr;       // fast launches' counter, then the slow launches'
+    uint32_t *punt_d = nullptr;            // punt count of each level, then one index slot per item
+    int slow_grid = 0;                     // workgroups of a slow launch over a punt list
     U128 *tmp_d = nullptr;
     size_t tmp_bytes = 0;
     bntt::R31 *twr_d = nullptr;          // digit copies of the distinct tables
@@ -155,6 +160,7 @@ void free_plan(bntt_batch_plan *p)
     (void)hipFree(p->descs_d);
     (void)hipFree(p->muls_d);
     (void)hipFree(p->ctl_d);
+    (void)hipFree(p->punt_d);
     (void)hipFree(p->tmp_d);
     (void)hipFree(p->twr_d);
     (void)hipFree(p->twflag_d);
@@ -165,18 +171,37 @@ void free_plan(bntt_batch_plan *p)
 
 int launch_levels(bntt_batch_plan *p, hipStream_t s)
 {
-    if (p->ntwjobs) {
-        hipLaunchKernelGGL(bntt::tw_flags_clear_kernel, dim3((p->ntwjobs + 255) / 256), dim3(256), 0, s, p->twflag_d,
-                           p->ntwjobs);
-        hipLaunchKernelGGL(bntt::tw_convert_kernel, dim3(p->ntwchunks), dim3(256), 0, s, p->twchunks_d, p->twjobs_d);
+    // each level: the fast launch, then a slow launch over the items it left to
+    // the literal formulas (usually none); a modulus not of the fast form runs
+    // every level on the literal formulas
+    const uint32_t nlev = (uint32_t)p->level_items.size();
+    const uint32_t nclear = std::max<uint32_t>(p->ntwjobs, nlev);
+    if (nclear) {
+        hipLaunchKernelGGL(bntt::exec_clear_kernel, dim3((nclear + 255) / 256), dim3(256), 0, s, p->twflag_d,
+                           p->ntwjobs, p->punt_d, nlev);
+        if (p->ntwjobs)
+            hipLaunchKernelGGL(bntt::tw_convert_kernel, dim3(p->ntwchunks), dim3(256), 0, s, p->twchunks_d,
+                               p->twjobs_d);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "twiddle conversion launch");
     }
-    for (size_t l = 0; l < p->level_items.size(); l++) {
+    for (size_t l = 0; l < nlev; l++) {
         if (p->level_items[l] == 0) continue;
-        hipLaunchKernelGGL(bntt::queue_kernel<false>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
-                           p->level_lds[l], s, p->items_d + p->level_offset[l], (uint32_t)p->level_items[l],
-                           p->descs_d, p->muls_d, p->ctl_d, p->level_tw[l], p->md);
+        const WorkItem *items = p->items_d + p->level_offset[l];
+        const uint32_t n = (uint32_t)p->level_items[l];
+        if (p->md.pm) {
+            const bntt::PuntList punt{p->punt_d + l, p->punt_d + nlev + p->level_offset[l]};
+            hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_FAST>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
+                               p->level_lds[l], s, items, n, p->descs_d, p->muls_d, p->ctl_d, p->level_tw[l], p->md,
+                               punt);
+            hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_SLOW>, dim3(p->slow_grid), dim3(bntt::kQueueThreads),
+                               p->level_lds[l], s, items, 0u, p->descs_d, p->muls_d, p->ctl_d + 1, bntt::TwArea{},
+                               p->md, punt);
+        } else {
+            hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_SLOW>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
+                               p->level_lds[l], s, items, n, p->descs_d, p->muls_d, p->ctl_d + 1, bntt::TwArea{},
+                               p->md, bntt::PuntList{nullptr, nullptr});
+        }
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "queue kernel launch");
     }
@@ -489,12 +514,15 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         grid_cap = prop.multiProcessorCount;
     }
     // dynamic LDS above 64 KiB must be allowed before occupancy is queried
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<false>),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)bntt::kQueueMaxLds);
-    if (attr != hipSuccess) {
+    static const hipError_t attr =
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_FAST>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds);
+    static const hipError_t attr_slow =
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_SLOW>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds);
+    if (attr != hipSuccess || attr_slow != hipSuccess) {
         free_plan(p);
-        return bntt_set_hip_error(attr, "hipFuncSetAttribute");
+        return bntt_set_hip_error(attr != hipSuccess ? attr : attr_slow, "hipFuncSetAttribute");
     }
     const size_t nlevels = 5;
     for (size_t l = 0; l < nlevels; l++) {
@@ -527,7 +555,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         bntt::TwArea area;
         size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), budget, &area);
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<false>, bntt::kQueueThreads,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<bntt::QK_FAST>, bntt::kQueueThreads,
                                                          lds) != hipSuccess || per_cu < 1)
             per_cu = 1;
         const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), (size_t)grid_cap * per_cu);
@@ -556,11 +584,17 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         free_plan(p);
         return rc;
     }
-    if (hipMalloc(&p->ctl_d, sizeof(bntt::QueueCtl)) != hipSuccess) {
+    // punt lists: a count per level, then one slot per item of the plan
+    const size_t punt_words = p->level_items.size() + p->nitems;
+    if (hipMalloc(&p->ctl_d, 2 * sizeof(bntt::QueueCtl)) != hipSuccess ||
+        hipMalloc(&p->punt_d, std::max<size_t>(punt_words, 1) * sizeof(uint32_t)) != hipSuccess) {
         free_plan(p);
         return bntt_set_error(BNTT_ENOMEM, "hipMalloc for queue counters failed");
     }
-    if (hipMemset(p->ctl_d, 0, sizeof(bntt::QueueCtl)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    p->slow_grid = grid_cap;  // one workgroup per CU: the literal formulas are the exception
+    if (hipMemset(p->ctl_d, 0, 2 * sizeof(bntt::QueueCtl)) != hipSuccess ||
+        hipMemset(p->punt_d, 0, std::max<size_t>(punt_words, 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
         free_plan(p);
         return bntt_set_error(BNTT_EHIP, "initialising queue counters failed");
     }

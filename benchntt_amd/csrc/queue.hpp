@@ -5,7 +5,17 @@
 // A launch processes one list of work items: a grid sized to the resident
 // capacity of the chip (at most one workgroup per item) whose workgroups take
 // items from an atomic counter until the list is exhausted; the last draw
-// rearms the counter for the next launch.  Kinds of item:
+// rearms the counter for the next launch.  Kinds of launch:
+//   QK_FAST  reference arithmetic on its fast path only (modulus 2^124 - c):
+//            an item that needs the literal formulas (a value or twiddle
+//            >= 2^124, or a product operand >= 2^124) is left untouched and
+//            its index appended to the launch's punt list
+//   QK_SLOW  the literal reference formulas for every item it takes: the
+//            punt list of the fast launch before it, or a whole list when the
+//            modulus is not of the fast form
+//   QK_MONT  exact Montgomery arithmetic (polynomial-multiply server)
+// Keeping the literal formulas out of the fast kernel keeps its registers
+// free of spills.  Kinds of item:
 //   ITEM_NTT   stages [S, S+k) of the forward network for G groups (several
 //              whole transforms when k = log2 N, else blocks of one transform)
 //   ITEM_MUL   2048 elements of c = a * b mod q
@@ -71,6 +81,15 @@ struct QueueCtl {
     uint32_t pad[3];
 };
 
+enum QueueKind : int { QK_FAST = 0, QK_SLOW = 1, QK_MONT = 2 };
+
+// Items a fast launch left to the literal formulas: count is cleared before
+// each execution, idx holds up to one entry per item of the launch's list.
+struct PuntList {
+    uint32_t *count;
+    uint32_t *idx;
+};
+
 // Conversion of twiddle tables to digits.
 struct TwJob {
     const U128 *src;
@@ -87,19 +106,20 @@ constexpr int kTwChunk = 4096;
 
 // klds > 0: the tables of the first klds stages are staged in LDS, the later
 // stages read the digit table in device memory.
-template <int ELOG, bool MONT>
-__device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, bool slow,
-                                                 int klds, const Mod &md)
+template <int ELOG, int KIND>
+__device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, int klds,
+                                                 const Mod &md)
 {
-    if constexpr (MONT) {
+    if constexpr (KIND == QK_MONT) {
         if (klds > 0)
             run_group_network<ELOG, TW_MONT_LDS, true, TW_MONT>(lds, tv, k, S, mbase, md, klds);
         else
             run_group_network<ELOG, TW_MONT, true>(lds, tv, k, S, mbase, md);
+    } else if constexpr (KIND == QK_SLOW) {
+        // the literal Barrett product whatever the modulus
+        run_group_network<ELOG, TW_SLOW, false>(lds, tv, k, S, mbase, md);
     } else {
-        if (slow)
-            run_group_network<ELOG, TW_SLOW, true>(lds, tv, k, S, mbase, md);
-        else if (klds > 0)
+        if (klds > 0)
             run_group_network<ELOG, TW_LDS, true, TW_GLOBAL>(lds, tv, k, S, mbase, md, klds);
         else
             run_group_network<ELOG, TW_GLOBAL, true>(lds, tv, k, S, mbase, md);
@@ -155,11 +175,13 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
     return log2n ? (__brev(i) >> (32u - log2n)) : 0u;
 }
 
-// One transform item: load, network, store.
-template <bool MONT>
-__device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, uint64_t *s_twkey,
+// One transform item: load, network, store.  Returns false when a fast
+// launch leaves the item to the literal formulas (nothing written).
+template <int KIND>
+__device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, uint64_t *s_twkey,
                             const WorkItem &it, const DevDesc *__restrict__ descs, TwArea area, const Mod &md)
 {
+    constexpr bool MONT = KIND == QK_MONT;
     const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
     const int G = 1 << Glog, E = 1 << elog;
     const int mlog = L - k;
@@ -239,14 +261,14 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         }
     }
     const uint32_t mbase = (uint32_t)it.first;
-    bool slow = false;
-    if constexpr (MONT) {
-        // the Montgomery path has no literal fallback: inputs must be canonical
+    if constexpr (KIND == QK_FAST) {
+        big |= *d0.twflag != 0;
+        if (__syncthreads_or(big)) return false;
+    } else {
+        // Montgomery: no literal fallback, inputs must be canonical; slow
+        // launches take the literal formulas whatever the values
         (void)big;
         __syncthreads();
-    } else {
-        big |= *d0.twflag != 0;
-        slow = __syncthreads_or(big) || md.pm == 0;
     }
     TwView tv;
     tv.lds = nullptr;
@@ -260,7 +282,7 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     // stage the tables of as many first stages as the launch's area holds
     // (at least one radix-8 sub-pass' worth)
     int klds = 0;
-    if (!slow)
+    if (KIND != QK_SLOW)
         for (int j = k; j >= 3; j--)
             if (area.tw_cap >= tw_slots(j, wlog)) {
                 klds = j;
@@ -283,13 +305,13 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
     }
     switch (elog) {
     case 1:
-        run_network_elog<1, MONT>(lds, tv, k, S, mbase, slow, klds, md);
+        run_network_elog<1, KIND>(lds, tv, k, S, mbase, klds, md);
         break;
     case 2:
-        run_network_elog<2, MONT>(lds, tv, k, S, mbase, slow, klds, md);
+        run_network_elog<2, KIND>(lds, tv, k, S, mbase, klds, md);
         break;
     default:
-        run_network_elog<3, MONT>(lds, tv, k, S, mbase, slow, klds, md);
+        run_network_elog<3, KIND>(lds, tv, k, S, mbase, klds, md);
         break;
     }
     // LDS -> HBM
@@ -323,28 +345,54 @@ __device__ void do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             }
         }
     }
+    return true;
 }
 
-__device__ __forceinline__ void do_mul_item(const WorkItem &it, const DevMul *__restrict__ muls, const Mod &md)
+// One product item (reference arithmetic).  A fast launch leaves an item with
+// an operand >= 2^124 to the literal formulas (returns false, nothing written).
+template <int KIND>
+__device__ __forceinline__ bool do_mul_item(const WorkItem &it, const DevMul *__restrict__ muls, const Mod &md)
 {
+    constexpr int J = kQueueTile / kQueueThreads;
     const DevMul mu = muls[it.desc];
+    if constexpr (KIND == QK_FAST) {
+        U128 a[J], b[J];
+        bool big = false;
 #pragma unroll
-    for (int j = 0; j < kQueueTile / kQueueThreads; j++) {
-        const uint32_t e = threadIdx.x + j * kQueueThreads;
-        if (e < it.nvalid) {
-            const size_t i = it.first + e;
-            const U128 a = ld16(mu.a + i), b = ld16(mu.b + i);
-            st16(mu.c + i, md.pm ? mulmod_elem<true>(a, b, md) : mulmod_elem<false>(a, b, md));
+        for (int j = 0; j < J; j++) {
+            const uint32_t e = threadIdx.x + j * kQueueThreads;
+            if (e < it.nvalid) {
+                a[j] = ld16(mu.a + it.first + e);
+                b[j] = ld16(mu.b + it.first + e);
+                big |= !below_2_124(a[j], b[j]);
+            }
+        }
+        if (__syncthreads_or(big)) return false;
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const uint32_t e = threadIdx.x + j * kQueueThreads;
+            if (e < it.nvalid) st16(mu.c + it.first + e, mulmod_r31(b[j], to_r31(a[j]), md.c));
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < J; j++) {
+            const uint32_t e = threadIdx.x + j * kQueueThreads;
+            if (e < it.nvalid) {
+                const size_t i = it.first + e;
+                st16(mu.c + i, mulmod_ref(ld16(mu.a + i), ld16(mu.b + i), md.m, md.mu));
+            }
         }
     }
+    return true;
 }
 
-template <bool MONT>
+template <int KIND>
 __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem *__restrict__ items, uint32_t nitems,
                                                                  const DevDesc *__restrict__ descs,
                                                                  const DevMul *__restrict__ muls, QueueCtl *ctl,
-                                                                 TwArea area, Mod md)
+                                                                 TwArea area, Mod md, PuntList punt)
 {
+    constexpr bool MONT = KIND == QK_MONT;
     extern __shared__ U128 lds[];
     __shared__ uint32_t s_ticket;
     __shared__ uint64_t s_twkey[2];  // tables staged in the twiddle area (none yet)
@@ -352,6 +400,10 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
     __shared__ U128 *s_y[kQueueThreads];
     __shared__ const U128 *s_x2[MONT ? kQueueThreads : 1];
     if (threadIdx.x == 0) s_twkey[0] = s_twkey[1] = ~0ull;
+    // a slow launch over a punt list takes the items the fast launch before
+    // it on the stream appended (that launch has completed)
+    if constexpr (KIND == QK_SLOW)
+        if (punt.idx) nitems = __hip_atomic_load(punt.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // Workgroup b starts with item b; after each item it draws from the
     // counter, draw c giving item gridDim + c.  A device-wide atomic is
     // performed beyond the XCDs' L2 caches, so the first round draws nothing.
@@ -360,13 +412,16 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
     // counter and rearms it for the next launch on this counter block.
     uint32_t t = blockIdx.x;
     while (t < nitems) {
-        const WorkItem it = items[t];
+        const WorkItem it = items[(KIND == QK_SLOW && punt.idx) ? punt.idx[t] : t];
+        bool done = true;
         if (it.kind == ITEM_NTT) {
-            do_ntt_item<MONT>(lds, s_x, s_y, s_x2, s_twkey, it, descs, area, md);
+            done = do_ntt_item<KIND>(lds, s_x, s_y, s_x2, s_twkey, it, descs, area, md);
         } else if constexpr (!MONT) {
-            do_mul_item(it, muls, md);
+            done = do_mul_item<KIND>(it, muls, md);
         }
         if (threadIdx.x == 0) {
+            if constexpr (KIND == QK_FAST)
+                if (!done) punt.idx[atomicAdd(punt.count, 1u)] = t;
             const uint32_t c = atomicAdd(&ctl->ticket, 1u);
             if (c == nitems - 1) atomicExch(&ctl->ticket, 0u);
             s_ticket = gridDim.x + c;

@@ -410,8 +410,9 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
         bntt::TwArea area;
         const size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), s->tw_budget, &area);
         const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)(lds > bntt::kQueueLds3 ? s->grid_cap2 : s->grid_cap));
-        hipLaunchKernelGGL(bntt::queue_kernel<true>, dim3(grid), dim3(bntt::kQueueThreads), lds, sl.stream,
-                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.ctl, area, s->md);
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT>, dim3(grid), dim3(bntt::kQueueThreads), lds, sl.stream,
+                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.ctl, area, s->md,
+                           bntt::PuntList{nullptr, nullptr});
         e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "serving kernel launch");
     }
@@ -503,16 +504,16 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
         }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, s->dev) != hipSuccess ||
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<true>),
+        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
         destroy_server(s);
         return bntt_set_error(BNTT_EHIP, "querying the device for the serving kernel failed");
     }
     int per_cu = 0, per_cu2 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<true>, bntt::kQueueThreads,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<bntt::QK_MONT>, bntt::kQueueThreads,
                                                      bntt::kQueueLds3) != hipSuccess)
         per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bntt::queue_kernel<true>, bntt::kQueueThreads,
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bntt::queue_kernel<bntt::QK_MONT>, bntt::kQueueThreads,
                                                      bntt::kQueueMaxLds) != hipSuccess)
         per_cu2 = 1;
     s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);

@@ -31,20 +31,20 @@ def build(torch, wl):
     return dx, dy, dc, tabs, tasks, muls
 
 
-def check_all(orc, wl, y, c, threads=16, every=1):
+def check_all(orc, wl, y, c, threads=16, every=1, q=QN, mu=MUN):
     """Every transform (or every `every`-th pair) and its product vs the oracle."""
     def one(j):
         outs = []
         for i in (2 * j, 2 * j + 1):
             l, off = int(wl.log2n[i]), int(wl.offsets[i])
             n = 1 << l
-            want = orc.ntt(wl.x[off:off + n], wl.tables[l], QN, MUN)
+            want = orc.ntt(wl.x[off:off + n], wl.tables[l], q, mu)
             if not np.array_equal(y[off:off + n], want):
                 return f"transform {i} (N=2^{l}) differs"
             outs.append(want)
         poff = int(wl.pair_offsets[j])
         n = outs[0].shape[0]
-        if not np.array_equal(c[poff:poff + n], orc.vvmul(outs[0], outs[1], QN, MUN)):
+        if not np.array_equal(c[poff:poff + n], orc.vvmul(outs[0], outs[1], q, mu)):
             return f"product {j} differs"
         return None
 
@@ -150,6 +150,50 @@ def test_edge_values_in_batch(gpu, orc):
         gpu.cuda.synchronize()
         check_all(orc, wl, from_dev(dy), from_dev(dc))
         plan.close()
+
+
+def test_literal_modulus(gpu, orc):
+    """A modulus outside the fast form 2^124 - c (q_116 with the reference's
+    MBITS=116 constant) runs every level on the literal formulas."""
+    from benchntt_amd.batch import BatchPlan
+
+    q = C.Q_116
+    mu = (1 << 235) // q
+    wl = uneven_pairs(40, q, seed=13, lo=1, hi=14)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, q, mu, graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc), q=q, mu=mu)
+    plan.close()
+
+
+def test_flagged_table_and_replays(gpu, orc):
+    """A twiddle table with an entry >= 2^124 sends every transform using it
+    (and the products of their outputs) to the literal launch after each
+    level; graph replays that alternate inputs with and without values >=
+    2^124 each match the oracle (the punt lists are cleared every execution)."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(40, QN, seed=17, lo=4, hi=15)
+    sizes = sorted({int(l) for l in wl.log2n})
+    wl.tables[sizes[-1]][3, 1] = np.uint64(0xFFFFFFFFFFFFFFFF)   # a two-pass size
+    wl.tables[sizes[1]][1, 1] |= np.uint64(1 << 61)              # a one-pass size
+    clean = wl.x.copy()
+    dirty = wl.x.copy()
+    dirty[11::193, 1] = np.uint64(0xFFFFFFFFFFFFFFFF)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=True)
+    stream = gpu.cuda.Stream()
+    for rep in range(4):
+        wl.x = dirty if rep % 2 == 0 else clean
+        dx.copy_(to_dev(gpu, wl.x))
+        gpu.cuda.synchronize()
+        with gpu.cuda.stream(stream):
+            plan.execute()
+        stream.synchronize()
+        check_all(orc, wl, from_dev(dy), from_dev(dc))
+    plan.close()
 
 
 def test_launch_each_matches(gpu, orc):
