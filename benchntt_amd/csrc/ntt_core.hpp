@@ -128,10 +128,12 @@ __device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFF
 //   TW_MONT_LDS: TW_MONT arithmetic with TW_LDS staged tables.
 //   TW_G128:   the caller's table, entries converted to digits in registers,
 //              fast arithmetic (stages left unstaged by a TW_LDS workgroup).
-enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2, TW_MONT = 3, TW_MONT_LDS = 4, TW_G128 = 5 };
+enum TwMode { TW_SLOW = 0, TW_LDS = 1, TW_GLOBAL = 2, TW_MONT = 3, TW_MONT_LDS = 4, TW_G128 = 5, TW_MONT_S = 6, TW_MONT_S_LDS = 7 };
 
-__host__ __device__ constexpr bool tw_in_lds(int mode) { return mode == TW_LDS || mode == TW_MONT_LDS; }
-__host__ __device__ constexpr bool tw_mont(int mode) { return mode == TW_MONT || mode == TW_MONT_LDS; }
+// Montgomery modes for a modulus whose 31-bit digits 1 and 2 are zero
+__host__ __device__ constexpr bool tw_sparse(int mode) { return mode == TW_MONT_S || mode == TW_MONT_S_LDS; }
+__host__ __device__ constexpr bool tw_in_lds(int mode) { return mode == TW_LDS || mode == TW_MONT_LDS || mode == TW_MONT_S_LDS; }
+__host__ __device__ constexpr bool tw_mont(int mode) { return mode == TW_MONT || mode == TW_MONT_LDS || tw_sparse(mode); }
 
 struct TwView {
     const R31 *lds;     // TW_LDS tables
@@ -190,7 +192,7 @@ __device__ __forceinline__ void reg_network_fast(U128 (&v)[1 << KP], const R31 (
 // v*tw, and lazy reduction (Harvey's butterfly): values stay below 4q; the
 // even input is brought below 2q, the product is below 2q, and the outputs
 // x + p and x + 2q - p are below 4q.  The caller makes the results canonical.
-template <int KP>
+template <int KP, bool SPARSE>
 __device__ __forceinline__ void reg_network_mont(U128 (&v)[1 << KP], const R31 (&t)[(1 << KP) - 1], const Mod &md)
 {
     constexpr int H = 1 << (KP - 1);
@@ -200,7 +202,7 @@ __device__ __forceinline__ void reg_network_mont(U128 (&v)[1 << KP], const R31 (
 #pragma unroll
         for (int i = 0; i < H; i++) {
             const U128 x = sub_if_ge(v[i], md.mq.ql2);
-            const U128 p = montmul_r31_lazy(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.mq);
+            const U128 p = montmul_r31_lazy<SPARSE>(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.mq);
             w[2 * i] = add128l(x, p);
             w[2 * i + 1] = add128l(x, sub128l(md.mq.ql2, p));
         }
@@ -281,7 +283,7 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
         const int g = sg >> sub_log;
         const int Mp = sg & ((1 << sub_log) - 1);
         if constexpr (tw_mont(MODE)) {
-            reg_network_mont<KP>(v[q], t[q], md);
+            reg_network_mont<KP, tw_sparse(MODE)>(v[q], t[q], md);
         } else if constexpr (MODE != TW_SLOW) {
             reg_network_fast<KP>(v[q], t[q], md);
         } else {

@@ -13,7 +13,9 @@
 //   QK_SLOW  the literal reference formulas for every item it takes: the
 //            punt list of the fast launch before it, or a whole list when the
 //            modulus is not of the fast form
-//   QK_MONT  exact Montgomery arithmetic (polynomial-multiply server)
+//   QK_MONT  exact Montgomery arithmetic (polynomial-multiply server);
+//            QK_MONT_S the same for a modulus whose 31-bit digits 1 and 2
+//            are zero (a shorter reduction)
 // Keeping the literal formulas out of the fast kernel keeps its registers
 // free of spills.  Kinds of item:
 //   ITEM_NTT   stages [S, S+k) of the forward network for G groups (several
@@ -81,7 +83,8 @@ struct QueueCtl {
     uint32_t pad[3];
 };
 
-enum QueueKind : int { QK_FAST = 0, QK_SLOW = 1, QK_MONT = 2 };
+enum QueueKind : int { QK_FAST = 0, QK_SLOW = 1, QK_MONT = 2, QK_MONT_S = 3 };
+__host__ __device__ constexpr bool qk_mont(int kind) { return kind == QK_MONT || kind == QK_MONT_S; }
 
 // Items a fast launch left to the literal formulas: count is cleared before
 // each execution, idx holds up to one entry per item of the launch's list.
@@ -110,11 +113,13 @@ template <int ELOG, int KIND>
 __device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase, int klds,
                                                  const Mod &md)
 {
-    if constexpr (KIND == QK_MONT) {
+    if constexpr (qk_mont(KIND)) {
+        constexpr int M = KIND == QK_MONT_S ? TW_MONT_S : TW_MONT;
+        constexpr int ML = KIND == QK_MONT_S ? TW_MONT_S_LDS : TW_MONT_LDS;
         if (klds > 0)
-            run_group_network<ELOG, TW_MONT_LDS, true, TW_MONT>(lds, tv, k, S, mbase, md, klds);
+            run_group_network<ELOG, ML, true, M>(lds, tv, k, S, mbase, md, klds);
         else
-            run_group_network<ELOG, TW_MONT, true>(lds, tv, k, S, mbase, md);
+            run_group_network<ELOG, M, true>(lds, tv, k, S, mbase, md);
     } else if constexpr (KIND == QK_SLOW) {
         // the literal Barrett product whatever the modulus
         run_group_network<ELOG, TW_SLOW, false>(lds, tv, k, S, mbase, md);
@@ -165,9 +170,10 @@ __device__ __forceinline__ U128 canonical(const U128 &a, const MontQ &mq)
 
 // A * B * R^-1 for forward outputs A, B below 4q (lazy butterflies): B is
 // brought below q, the product is below 2q (a valid network input).
+template <bool SPARSE>
 __device__ __forceinline__ U128 gather_product(const U128 &a, const U128 &b, const MontQ &mq)
 {
-    return montmul_r31_lazy(a, to_r31(canonical(b, mq)), mq);
+    return montmul_r31_lazy<SPARSE>(a, to_r31(canonical(b, mq)), mq);
 }
 
 __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
@@ -181,7 +187,7 @@ template <int KIND>
 __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, uint64_t *s_twkey,
                             const WorkItem &it, const DevDesc *__restrict__ descs, TwArea area, const Mod &md)
 {
-    constexpr bool MONT = KIND == QK_MONT;
+    constexpr bool MONT = qk_mont(KIND), SPARSE = KIND == QK_MONT_S;
     const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
     const int G = 1 << Glog, E = 1 << elog;
     const int mlog = L - k;
@@ -231,7 +237,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                 if (gather) {
                     // network input brv(j) is A[j] * B[j]: contiguous loads,
                     // bit-reversed placement in LDS
-                    const U128 a = gather_product(ld16(s_x[g] + ti), gmul ? ld16(s_x2[g] + ti) : one, md.mq);
+                    const U128 a = gather_product<SPARSE>(ld16(s_x[g] + ti), gmul ? ld16(s_x2[g] + ti) : one, md.mq);
                     lds[lidx((g << k) + (int)bitrev((uint32_t)ti, (uint32_t)k), k)] = a;
                 } else {
                     const U128 a = ld16(s_x[g] + ti);
@@ -246,7 +252,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             const int g = e >> k, j = e & ((1 << k) - 1);
             if (g < nvalid) {
                 const uint32_t blk = bitrev((uint32_t)it.first + (uint32_t)g, (uint32_t)mlog) << k;
-                const U128 a = gather_product(ld16(d0.x + blk + j), gmul ? ld16(d0.x2 + blk + j) : one, md.mq);
+                const U128 a = gather_product<SPARSE>(ld16(d0.x + blk + j), gmul ? ld16(d0.x2 + blk + j) : one, md.mq);
                 lds[lidx((g << k) + (int)bitrev((uint32_t)j, (uint32_t)k), k)] = a;
             }
         } else {
@@ -327,7 +333,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                     // position ti of c takes network output brv(ti): contiguous
                     // stores, factor from the table permuted to positions
                     const U128 y = lds[lidx((g << k) + (int)bitrev(ti, (uint32_t)k), k)];
-                    st16(s_y[g] + ti, montmul_r31(y, d0.scale[((size_t)1 << k) + ti], md.mq));
+                    st16(s_y[g] + ti, montmul_r31<SPARSE>(y, d0.scale[((size_t)1 << k) + ti], md.mq));
                 } else {
                     const U128 y = lds[lidx(e, k)];
                     st16(s_y[g] + ti, canon ? canonical(y, md.mq) : y);
@@ -337,7 +343,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             if ((e >> k) < nvalid) {
                 const uint32_t idx = ((uint32_t)it.first << k) + (uint32_t)e;
                 if (scatter)
-                    st16(dst + bitrev(idx, (uint32_t)L), montmul_r31(lds[lidx(e, k)], d0.scale[idx], md.mq));
+                    st16(dst + bitrev(idx, (uint32_t)L), montmul_r31<SPARSE>(lds[lidx(e, k)], d0.scale[idx], md.mq));
                 else if (canon)
                     st16(dst + idx, canonical(lds[lidx(e, k)], md.mq));
                 else
@@ -392,7 +398,7 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
                                                                  const DevMul *__restrict__ muls, QueueCtl *ctl,
                                                                  TwArea area, Mod md, PuntList punt)
 {
-    constexpr bool MONT = KIND == QK_MONT;
+    constexpr bool MONT = qk_mont(KIND);
     extern __shared__ U128 lds[];
     __shared__ uint32_t s_ticket;
     __shared__ uint64_t s_twkey[2];  // tables staged in the twiddle area (none yet)

@@ -172,7 +172,22 @@ struct bntt_pm_server {
     int grid_cap = 0;         // resident workgroups with up to kQueueLds3 of dynamic LDS
     int grid_cap2 = 0;        // ... with up to kQueueMaxLds
     size_t tw_budget = 0;     // twiddles from device memory (staging measured slower)
+    bool sparse = false;      // q's 31-bit digits 1 and 2 are zero: the shorter Montgomery reduction
 };
+
+namespace {
+
+// The serving kernel for the server's modulus.
+template <typename... A>
+void launch_serving(const bntt_pm_server *s, dim3 grid, size_t lds, hipStream_t st, A... args)
+{
+    if (s->sparse)
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT_S>, grid, dim3(bntt::kQueueThreads), lds, st, args...);
+    else
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT>, grid, dim3(bntt::kQueueThreads), lds, st, args...);
+}
+
+}  // namespace
 
 namespace {
 
@@ -410,9 +425,10 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
         bntt::TwArea area;
         const size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), s->tw_budget, &area);
         const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)(lds > bntt::kQueueLds3 ? s->grid_cap2 : s->grid_cap));
-        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT>, dim3(grid), dim3(bntt::kQueueThreads), lds, sl.stream,
-                           sl.items_d + off[l], (uint32_t)lv[l].size(), sl.descs_d, nullptr, sl.ctl, area, s->md,
-                           bntt::PuntList{nullptr, nullptr});
+        launch_serving(s, dim3(grid), lds, sl.stream, static_cast<const WorkItem *>(sl.items_d + off[l]),
+                       (uint32_t)lv[l].size(), static_cast<const DevDesc *>(sl.descs_d),
+                       static_cast<const bntt::DevMul *>(nullptr), sl.ctl, area, s->md,
+                       bntt::PuntList{nullptr, nullptr});
         e = hipGetLastError();
         if (e != hipSuccess) return bntt_set_hip_error(e, "serving kernel launch");
     }
@@ -502,19 +518,20 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
             destroy_server(s);
             return rc;
         }
+    s->sparse = s->md.mq.q.d[1] == 0 && s->md.mq.q.d[2] == 0;
+    const void *kfn = s->sparse ? reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT_S>)
+                                : reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT>);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, s->dev) != hipSuccess ||
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
+        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
         destroy_server(s);
         return bntt_set_error(BNTT_EHIP, "querying the device for the serving kernel failed");
     }
     int per_cu = 0, per_cu2 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<bntt::QK_MONT>, bntt::kQueueThreads,
-                                                     bntt::kQueueLds3) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, bntt::kQueueThreads, bntt::kQueueLds3) != hipSuccess)
         per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, bntt::queue_kernel<bntt::QK_MONT>, bntt::kQueueThreads,
-                                                     bntt::kQueueMaxLds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, kfn, bntt::kQueueThreads, bntt::kQueueMaxLds) !=
+        hipSuccess)
         per_cu2 = 1;
     s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);
     s->grid_cap2 = prop.multiProcessorCount * std::max(per_cu2, 1);

@@ -13,7 +13,8 @@ extern "C" {
 // op: 0 addmod_ref, 1 submod_ref, 2 mulmod_ref, 3 mulmod_pm, 4 mulmod<true>, 5 mulmod<false>,
 //     6 mulmod_r31 (b converted to digits first), 7 montmul_r31 (mu = -q^-1 mod 2^124),
 //     8 addmod_canon, 9 submod_canon, 10 montmul_r31_lazy (mu = -q^-1 mod 2^124),
-//     11 one lazy Montgomery butterfly of x = a, y = b with w = c (out[2i], out[2i+1])
+//     11 one lazy Montgomery butterfly of x = a, y = b with w = c (out[2i], out[2i+1]),
+//     12 montmul_r31_lazy<SPARSE> (q's 31-bit digits 1 and 2 zero)
 void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U128 *m, const U128 *mu, uint32_t c)
 {
     bntt::Mod md;
@@ -41,7 +42,8 @@ void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U12
         }
         case 8: out[i] = bntt::addmod_canon(a[i], b[i], *m); break;
         case 10:
-        case 11: {
+        case 11:
+        case 12: {
             bntt::MontQ mq;
             mq.q = bntt::to_r31(*m);
             mq.qinv = bntt::to_r31(*mu);
@@ -49,6 +51,8 @@ void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U12
             mq.ql2 = bntt::add128l(*m, *m);
             if (op == 10) {
                 out[i] = bntt::montmul_r31_lazy(a[i], bntt::to_r31(b[i]), mq);
+            } else if (op == 12) {
+                out[i] = bntt::montmul_r31_lazy<true>(a[i], bntt::to_r31(b[i]), mq);
             } else {
                 // the butterfly of reg_network_mont on (x = a, y = b), twiddle digit 0 = c
                 bntt::R31 w{};
