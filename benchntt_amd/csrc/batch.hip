@@ -69,7 +69,7 @@ __global__ __launch_bounds__(256) void tw_convert_kernel(const TwChunk *__restri
     for (uint32_t e = threadIdx.x; e < ch.count; e += 256) {
         const uint64_t i = ch.off + e;
         const U128 a = ld16(jb.src + i);
-        big |= (i != 0) && is_big(a);  // entry 0 is never read by the network
+        big |= (i != 0) && tw_big(a);  // entry 0 is never read by the network
         jb.dst[i] = to_r31(a);
     }
     if (__syncthreads_or(big) && threadIdx.x == 0 && jb.flag) atomicOr(jb.flag, 1u);

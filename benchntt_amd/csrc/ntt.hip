@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
             const int e = threadIdx.x + j * T;
             if ((e >> k) < nvalid) {
                 const U128 a = ld16(xb + e);
-                big |= is_big(a);
+                big |= near_q(a, md.m);
                 lds[lidx(e, k)] = a;
             }
         }
@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
                 const size_t tr = gamma >> mlog;
                 const size_t M = gamma & (((size_t)1 << mlog) - 1);
                 const U128 a = ld16(x + (tr << L) + ((size_t)ti << mlog) + M);
-                big |= is_big(a);
+                big |= near_q(a, md.m);
                 lds[lidx((g << k) + ti, k)] = a;
             }
         }
@@ -61,8 +61,12 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     const uint32_t W = 1u << wlog;
     const uint32_t P0start = (wlog == Glog) ? (mbase & ((1u << S) - 1u)) : 0u;
     R31 *tws = reinterpret_cast<R31 *>(lds + lds_data_slots(k, Glog));
-    big |= stage_twiddles(tws, tw, S, k, wlog, P0start, klds);
-    const bool slow = __syncthreads_or(big) || !PM;
+    const bool twbig = stage_twiddles(tws, tw, S, k, wlog, P0start, klds);
+    // the vote is also the barrier after the tile and table writes; a value
+    // near q (rare for residues) is decided on the exact values
+    bool slow = __syncthreads_or(big || twbig);
+    if (slow && PM) slow = __syncthreads_or(twbig) || tile_above_q(lds, nvalid << k, k, md.m);
+    slow = slow || !PM;
 
     TwView tv;
     tv.lds = tws;

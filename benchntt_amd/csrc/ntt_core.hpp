@@ -106,13 +106,37 @@ __host__ __device__ inline size_t lds_data_slots(int k, int Glog)
     return n + n / 8 + ((size_t)1 << Glog);
 }
 
-// A value is "small" when its top limb is below that of q = 2^124 - c for
-// every c < 2^16 (0x0FFFFFFF): then it is below q.  When every input and
-// twiddle of a workgroup is small, every value the network produces is at
-// most q (< 2^124), so the fast exact reduction is valid for all its products
-// and equals the reference's Barrett result; otherwise the workgroup runs the
-// literal reference formulas.
-__device__ __forceinline__ bool is_big(const U128 &a) { return a.v[3] >= 0x0FFFFFFFu; }
+// The fast path's conditions.  Its product equals the reference's Barrett
+// result for operands below 2^124.  When every input of a workgroup is at most
+// q and every twiddle is below 2^124, every value the network produces is at
+// most q (addmod and submod of values <= q, and products below q, stay <= q),
+// so all its products qualify; otherwise the workgroup runs the literal
+// reference formulas.
+//   near_q:  a's top limb reaches q's (necessary for a > q; one compare, made
+//            as values are loaded)
+//   above_q: a > q exactly (run over a workgroup's tile only when some value
+//            was near q)
+//   tw_big:  a >= 2^124.
+__device__ __forceinline__ bool near_q(const U128 &a, const U128 &q) { return a.v[3] >= q.v[3]; }
+__device__ __forceinline__ bool above_q(const U128 &a, const U128 &q)
+{
+    uint32_t b;
+    (void)subb(q.v[0], a.v[0], 0, b);
+    (void)subb(q.v[1], a.v[1], b, b);
+    (void)subb(q.v[2], a.v[2], b, b);
+    (void)subb(q.v[3], a.v[3], b, b);
+    return b != 0;
+}
+__device__ __forceinline__ bool tw_big(const U128 &a) { return (a.v[3] >> 28) != 0; }
+
+// Whether a workgroup whose loads flagged near (any lane, near_q) holds a
+// value above q: the exact test over the n values of its tile in LDS.
+__device__ __forceinline__ bool tile_above_q(const U128 *lds, int n, int k, const U128 &q)
+{
+    bool big = false;
+    for (int p = threadIdx.x; p < n; p += blockDim.x) big |= above_q(lds[lidx(p, k)], q);
+    return __syncthreads_or(big);
+}
 
 // Where a register sub-pass finds its twiddles.
 //   TW_LDS:    staged digit tables in LDS.  Stage table r holds, for the W
@@ -212,7 +236,7 @@ __device__ __forceinline__ void reg_network_mont(U128 (&v)[1 << KP], const R31 (
 }
 
 // The same stages with the caller's table and the literal reference formulas
-// (used when some operand of the workgroup is not small, or the modulus is not
+// (used when some input of the workgroup is above q or a twiddle >= 2^124, or the modulus is not
 // of the fast form).  Local stages r0 .. r0+KP-1 of a group network that starts
 // at global stage S; P0: the group's prefix (low S bits of its M); Mlow: low r0
 // bits of the sub-group index.
@@ -353,7 +377,7 @@ __device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, i
 // Stage, as digits, the twiddles that stages [S, S+klds) use for W = 2^wlog
 // consecutive group prefixes starting at P0start: stage table r (2^r entries
 // per prefix) sits at LDS offset W*(2^r - 1).  Returns whether any entry of
-// stages [S, S+k) is not small.
+// stages [S, S+k) is >= 2^124.
 __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict__ tw, int S, int k, int wlog,
                                                uint32_t P0start, int klds)
 {
@@ -365,7 +389,7 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
         // the table itself from entry 1 on
         for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
             const U128 a = ld16(tw + 1 + e);
-            big |= is_big(a);
+            big |= tw_big(a);
             if (e < nstaged) tws[tidx(e)] = to_r31(a);
         }
         return big;
@@ -376,14 +400,14 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
         const int r = 31 - __builtin_clz(u);
         const uint32_t off = (uint32_t)e - (((1u << r) - 1u) << wlog);
         const U128 a = ld16(tw + (1u << (S + r)) + (P0start << r) + off);
-        big |= is_big(a);
+        big |= tw_big(a);
         if (e < nstaged) tws[tidx(e)] = to_r31(a);
     }
     return big;
 }
 
 // The same staging from a table already in digits (no check: the caller
-// knows whether the table is small).
+// knows whether the table's entries are below 2^124).
 __device__ __forceinline__ void stage_twiddles_r31(R31 *tws, const R31 *__restrict__ twr, int S, int klds, int wlog,
                                                    uint32_t P0start)
 {

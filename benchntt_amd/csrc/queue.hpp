@@ -55,7 +55,7 @@ struct DevDesc {
     U128 *tmp;
     const U128 *tw;
     const R31 *twr;          // digit copy of tw (reference arithmetic) or tw*R digits (Montgomery)
-    const uint32_t *twflag;  // nonzero when some entry of tw (past entry 0) is not small
+    const uint32_t *twflag;  // nonzero when some entry of tw (past entry 0) is >= 2^124
     const U128 *x2;          // IT_GATHER_MUL: second factor B (x holds A)
     const R31 *scale;        // IT_SCATTER_SCALE: S in digits
 };
@@ -228,7 +228,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         if (contig) {
             if ((e >> k) < nvalid) {
                 const U128 a = ld16(d0.x + e);
-                big |= is_big(a);
+                big |= near_q(a, md.m);
                 lds[lidx(e, k)] = a;
             }
         } else if (mlog == 0) {
@@ -241,7 +241,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                     lds[lidx((g << k) + (int)bitrev((uint32_t)ti, (uint32_t)k), k)] = a;
                 } else {
                     const U128 a = ld16(s_x[g] + ti);
-                    big |= is_big(a);
+                    big |= near_q(a, md.m);
                     lds[lidx(e, k)] = a;
                 }
             }
@@ -261,15 +261,18 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                 // indices within one transform (at most 2^24 points) fit 32 bits
                 const uint32_t idx = ((uint32_t)ti << mlog) + (uint32_t)it.first + (uint32_t)g;
                 const U128 a = ld16(src + idx);
-                big |= is_big(a);
+                big |= near_q(a, md.m);
                 lds[lidx((g << k) + ti, k)] = a;
             }
         }
     }
     const uint32_t mbase = (uint32_t)it.first;
     if constexpr (KIND == QK_FAST) {
-        big |= *d0.twflag != 0;
-        if (__syncthreads_or(big)) return false;
+        // a large twiddle, or a value above q (decided exactly in the rare
+        // workgroups where some value is near q): the literal formulas.  The
+        // vote is also the barrier after the tile writes.
+        const bool twb = *d0.twflag != 0;  // one flag for the whole item
+        if (__syncthreads_or(big || twb) && (twb || tile_above_q(lds, nvalid << k, k, md.m))) return false;
     } else {
         // Montgomery: no literal fallback, inputs must be canonical; slow
         // launches take the literal formulas whatever the values

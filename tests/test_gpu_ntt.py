@@ -102,6 +102,31 @@ def test_products_near_zero(gpu, orc, L):
     assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
 
 
+def test_inputs_just_below_q(gpu, orc):
+    """Residues in [q - 2^100, q] (top limb equal to q's) take the fast path,
+    whose values then stay at most q; one launch and a batch plan both match
+    the oracle, with q itself among the inputs."""
+    import random
+
+    from benchntt_amd.batch import BatchPlan
+    from benchntt_amd.residues import from_ints
+
+    rng = random.Random(5)
+    n = 1024
+    x = from_ints([QN - rng.randrange(1 << 100) if j % 3 else QN - (j % 7) for j in range(4 * n)])
+    tw = uniform(n, QN, 77)
+    want = orc.ntt(x, tw, QN, MUN)
+    assert np.array_equal(ntt_dev(gpu, x, tw), want)
+    dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+    dy = gpu.empty_like(dx)
+    plan = BatchPlan([(dx[t * n:(t + 1) * n], dy[t * n:(t + 1) * n], dtw) for t in range(4)], [], QN, MUN,
+                     graph=False)
+    plan.execute()
+    gpu.cuda.synchronize()
+    assert np.array_equal(from_dev(dy), want)
+    plan.close()
+
+
 def test_other_modulus_literal_path(gpu, orc):
     q = C.Q_116
     mu = (1 << 235) // q
