@@ -14,14 +14,16 @@
 // with ordinary loads.  vvmul/axpy add one mulmod per element (about 75 VALU
 // ops, 68 VGPRs); with the wave-uniform fast path, axpy moved 6.7-6.9 TB/s in
 // 512-thread blocks and 7.0 TB/s in 256-thread blocks (1024: 6.7 TB/s; two
-// or four elements per lane: 6.8-6.9 TB/s), so all four use 256.
+// or four elements per lane: 6.8-6.9 TB/s).  Alternating 64/128/256-thread
+// blocks on three boxes, 128 was 0.5-1% ahead of 256 for every op (64: 1%
+// behind), so all four use 128.
 #include "kernels.hpp"
 
 namespace bntt {
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
-constexpr int kBlasThreads = 256;
+constexpr int kBlasThreads = 128;
 constexpr int kPerLane = 1;
 
 __device__ __forceinline__ U128 load_nt(const U128 *p)
