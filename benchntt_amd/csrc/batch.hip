@@ -514,15 +514,15 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         grid_cap = prop.multiProcessorCount;
     }
     // dynamic LDS above 64 KiB must be allowed before occupancy is queried
-    static const hipError_t attr =
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_FAST>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds);
-    static const hipError_t attr_slow =
-        hipFuncSetAttribute(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_SLOW>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds);
-    if (attr != hipSuccess || attr_slow != hipSuccess) {
+    static std::atomic<uint64_t> lds_fast{0}, lds_slow{0};
+    hipError_t attr = bntt_allow_lds(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_FAST>),
+                                     (int)bntt::kQueueMaxLds, lds_fast);
+    if (attr == hipSuccess)
+        attr = bntt_allow_lds(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_SLOW>),
+                              (int)bntt::kQueueMaxLds, lds_slow);
+    if (attr != hipSuccess) {
         free_plan(p);
-        return bntt_set_hip_error(attr != hipSuccess ? attr : attr_slow, "hipFuncSetAttribute");
+        return bntt_set_hip_error(attr, "hipFuncSetAttribute");
     }
     const size_t nlevels = 5;
     for (size_t l = 0; l < nlevels; l++) {

@@ -2,6 +2,7 @@
 // one twiddle table, in one launch (N <= 4096) or ceil(log2(N)/8) launches
 // through device memory (N > 4096).  The network and its blocking are
 // described in ntt_core.hpp.
+#include "internal.hpp"
 #include "ntt_core.hpp"
 
 namespace bntt {
@@ -103,10 +104,10 @@ static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int
     const size_t lds_bytes = pass_lds_bytes(k, S, Glog, klds);
     if (threads < 1 || threads > kMaxThreads || blocks > 0x7fffffffu || lds_bytes > kMaxLdsBytes)
         return hipErrorInvalidConfiguration;
-    // raise the dynamic LDS limit once per instantiation (thread-safe static init)
-    static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void *>(&ntt_pass_kernel<ELOG, PM>),
-                                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                                       (int)kMaxLdsBytes);
+    // raise the dynamic LDS limit once per instantiation and device
+    static std::atomic<uint64_t> lds_done{0};
+    const hipError_t attr =
+        bntt_allow_lds(reinterpret_cast<const void *>(&ntt_pass_kernel<ELOG, PM>), (int)kMaxLdsBytes, lds_done);
     if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL((ntt_pass_kernel<ELOG, PM>), dim3((unsigned)blocks), dim3(threads), lds_bytes, stream, y, x,
                        tw, L, S, k, Glog, klds, total_groups, md);
