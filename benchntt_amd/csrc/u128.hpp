@@ -352,7 +352,7 @@ BNTT_HD U128 mulmod_r31(const U128 &a_, const R31 &w, uint32_t c)
 // subtraction: for a < 4q and w < q the result is below 2q (T + m q < 4q^2 +
 // 2^124 q, and 4q < 2^124), which lazy butterflies keep as is.  SPARSE: q's
 // digits 1 and 2 are zero (q = d0 + d3 2^93, such as 2^115 + 47 2^17 + 1), so
-// m*q takes 8 multiply-adds instead of 16 and m is found digit by digit.
+// m*q takes 8 multiply-adds instead of 16.
 template <bool SPARSE = false>
 BNTT_HD U128 montmul_r31_lazy(const U128 &a_, const R31 &w, const MontQ &mq)
 {
@@ -383,64 +383,42 @@ BNTT_HD U128 montmul_r31_lazy(const U128 &a_, const R31 &w, const MontQ &mq)
     t = mad32(a.d[3], w.d[3], t >> 31);
     d[6] = (uint32_t)t & kM31;
     d[7] = (uint32_t)(t >> 31);
+    // u = t + m q with m = -t q^-1 mod R, found digit by digit: m_i =
+    // c_i (-q^-1 mod 2^31) mod 2^31 clears digit i of the running sum (the
+    // unique m < R that clears the low four digits).  SPARSE: q1 = q2 = 0.
     const R31 &q = mq.q;
-    const R31 &qi = mq.qinv;
+    const uint32_t qi0 = mq.qinv.d[0];
     uint32_t m[4], r[4];
-    if constexpr (SPARSE) {
-        // digit by digit: m_i = c_i * (-q^-1 mod 2^31) mod 2^31 clears digit i
-        // of t + m q (the same m as below: the unique m < R with t + m q = 0
-        // mod R); q contributes q0 at digit i and q3 at digit i + 3
-        uint64_t c = d[0];
-        m[0] = ((uint32_t)c * qi.d[0]) & kM31;
-        c = mad32(m[0], q.d[0], c);
-        c = (c >> 31) + d[1];
-        m[1] = ((uint32_t)c * qi.d[0]) & kM31;
-        c = mad32(m[1], q.d[0], c);
-        c = (c >> 31) + d[2];
-        m[2] = ((uint32_t)c * qi.d[0]) & kM31;
-        c = mad32(m[2], q.d[0], c);
-        c = mad32(m[0], q.d[3], (c >> 31) + d[3]);
-        m[3] = ((uint32_t)c * qi.d[0]) & kM31;
-        c = mad32(m[3], q.d[0], c);
-        t = mad32(m[1], q.d[3], (c >> 31) + d[4]);
-        r[0] = (uint32_t)t & kM31;
-        t = mad32(m[2], q.d[3], (t >> 31) + d[5]);
-        r[1] = (uint32_t)t & kM31;
-    } else {
-        // m = (t mod R) * qinv mod R
-        t = (uint64_t)d[0] * qi.d[0];
-        m[0] = (uint32_t)t & kM31;
-        t = mad32(d[0], qi.d[1], t >> 31);
-        t = mad32(d[1], qi.d[0], t);
-        m[1] = (uint32_t)t & kM31;
-        t = mad32(d[0], qi.d[2], t >> 31);
-        t = mad32(d[1], qi.d[1], t);
-        t = mad32(d[2], qi.d[0], t);
-        m[2] = (uint32_t)t & kM31;
-        t = mad32(d[0], qi.d[3], t >> 31);
-        t = mad32(d[1], qi.d[2], t);
-        t = mad32(d[2], qi.d[1], t);
-        t = mad32(d[3], qi.d[0], t);
-        m[3] = (uint32_t)t & kM31;
-        // u = t + m*q: the low four digits vanish, keep their carries
-        t = mad32(m[0], q.d[0], d[0]);
-        t = mad32(m[0], q.d[1], (t >> 31) + d[1]);
-        t = mad32(m[1], q.d[0], t);
-        t = mad32(m[0], q.d[2], (t >> 31) + d[2]);
-        t = mad32(m[1], q.d[1], t);
-        t = mad32(m[2], q.d[0], t);
-        t = mad32(m[0], q.d[3], (t >> 31) + d[3]);
-        t = mad32(m[1], q.d[2], t);
-        t = mad32(m[2], q.d[1], t);
-        t = mad32(m[3], q.d[0], t);
-        t = mad32(m[1], q.d[3], (t >> 31) + d[4]);
+    uint64_t c = d[0];
+    m[0] = ((uint32_t)c * qi0) & kM31;
+    c = mad32(m[0], q.d[0], c);
+    c = (c >> 31) + d[1];
+    if constexpr (!SPARSE) c = mad32(m[0], q.d[1], c);
+    m[1] = ((uint32_t)c * qi0) & kM31;
+    c = mad32(m[1], q.d[0], c);
+    c = (c >> 31) + d[2];
+    if constexpr (!SPARSE) {
+        c = mad32(m[0], q.d[2], c);
+        c = mad32(m[1], q.d[1], c);
+    }
+    m[2] = ((uint32_t)c * qi0) & kM31;
+    c = mad32(m[2], q.d[0], c);
+    c = mad32(m[0], q.d[3], (c >> 31) + d[3]);
+    if constexpr (!SPARSE) {
+        c = mad32(m[1], q.d[2], c);
+        c = mad32(m[2], q.d[1], c);
+    }
+    m[3] = ((uint32_t)c * qi0) & kM31;
+    c = mad32(m[3], q.d[0], c);
+    t = mad32(m[1], q.d[3], (c >> 31) + d[4]);
+    if constexpr (!SPARSE) {
         t = mad32(m[2], q.d[2], t);
         t = mad32(m[3], q.d[1], t);
-        r[0] = (uint32_t)t & kM31;
-        t = mad32(m[2], q.d[3], (t >> 31) + d[5]);
-        t = mad32(m[3], q.d[2], t);
-        r[1] = (uint32_t)t & kM31;
     }
+    r[0] = (uint32_t)t & kM31;
+    t = mad32(m[2], q.d[3], (t >> 31) + d[5]);
+    if constexpr (!SPARSE) t = mad32(m[3], q.d[2], t);
+    r[1] = (uint32_t)t & kM31;
     t = mad32(m[3], q.d[3], (t >> 31) + d[6]);
     r[2] = (uint32_t)t & kM31;
     r[3] = (uint32_t)(t >> 31) + d[7];  // u/R < 2q < 2^123: fits, no carry out
