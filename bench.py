@@ -187,6 +187,17 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     out["ntt_plain_1024_butterflies_per_s"] = round(5120 / (ms1 * 1e-3), 1)
     # INT-op/s = butterflies/s x the network's VALU lane-instructions per butterfly
     out["ntt_plain_1024_int_ops_per_s"] = round(5120 / (ms1 * 1e-3) * VALU_PER_BUTTERFLY, 1)
+    # the reference's call shape: host arrays in, host array out, one call at a
+    # time (PCIe and synchronisation included; the library's zero-copy path)
+    yh = np.empty_like(x_np)
+    for _ in range(20):
+        bn.ntt(yh, x_np, qn, mun, tw_np)
+    t0 = time.perf_counter()
+    for _ in range(200):
+        bn.ntt(yh, x_np, qn, mun, tw_np)
+    out["ntt_plain_1024_host_entry_us"] = round((time.perf_counter() - t0) / 200 * 1e6, 2)
+    log(f"ntt plain 1024: {ms1 * 1e3:.2f} us per launch on device arrays, "
+        f"{out['ntt_plain_1024_host_entry_us']} us per host-array call")
     tune = load_tune_table()
     total = 1 << 23
     xb = torch.from_numpy(uniform(total, qn, 5).view(np.int64)).to(dev)
@@ -214,7 +225,6 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
             out[f"{key}_persistent_vs_per_launch"] = round(mse / msp, 2)
         log(f"uniform {cnt}x{n}: persistent {msp:.4f} ms ({out[f'{key}_NTTs_per_s']:.0f} NTT/s)"
             + (f", one launch {msd:.4f} ms, per-transform launches {mse:.2f} ms" if L == 10 else ""))
-    log(f"ntt plain 1024: {ms1 * 1e3:.2f} us")
     if cpu:
         out.update(ntt_cpu_rows(np, x_np, tw_np, qn, mun))
     return out

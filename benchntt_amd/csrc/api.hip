@@ -127,16 +127,25 @@ int blas_dev(int op, void *c, const void *a, const void *b, const bntt_u128 *x, 
     return BNTT_OK;
 }
 
-// Library-owned staging for the host-pointer entry points: one stream and one
-// grow-only device arena per process (current device at first use).
+// Library-owned staging for the host-pointer entry points: one stream, one
+// grow-only device arena and one grow-only pinned host buffer per process
+// (current device at first use).  Small calls run zero-copy: the caller's
+// arrays are copied into the pinned buffer, the kernel reads and writes it
+// over PCIe, and the result is copied out, with no DMA transfer.
 struct Staging {
     std::mutex mu;
     hipStream_t stream = nullptr;
     void *buf = nullptr;
     size_t cap = 0;
+    void *hbuf = nullptr;  // pinned host memory
+    void *hdev = nullptr;  // its address for kernels
+    size_t hcap = 0;
     int dev = -1;
 };
 Staging g_stage;
+
+// Largest arrays (bytes each) that the host-pointer entry points move zero-copy.
+constexpr size_t kZeroCopyBytes = (size_t)4096 * sizeof(bntt_u128);
 
 int stage_reserve(size_t bytes)
 {
@@ -149,6 +158,8 @@ int stage_reserve(size_t bytes)
         g_stage.stream = nullptr;
         g_stage.buf = nullptr;
         g_stage.cap = 0;
+        g_stage.hbuf = g_stage.hdev = nullptr;
+        g_stage.hcap = 0;
         g_stage.dev = dev;
     }
     if (!g_stage.stream) {
@@ -170,6 +181,30 @@ int stage_reserve(size_t bytes)
     return BNTT_OK;
 }
 
+// The pinned host buffer, at least `bytes` (after stage_reserve on this device).
+int stage_host_reserve(size_t bytes)
+{
+    if (g_stage.hcap >= bytes) return BNTT_OK;
+    hipError_t e;
+    if (g_stage.hbuf) {
+        e = hipStreamSynchronize(g_stage.stream);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        (void)hipHostFree(g_stage.hbuf);
+        g_stage.hbuf = g_stage.hdev = nullptr;
+        g_stage.hcap = 0;
+    }
+    e = hipHostMalloc(&g_stage.hbuf, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    e = hipHostGetDevicePointer(&g_stage.hdev, g_stage.hbuf, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(g_stage.hbuf);
+        g_stage.hbuf = nullptr;
+        return hip_fail(e, "hipHostGetDevicePointer");
+    }
+    g_stage.hcap = bytes;
+    return BNTT_OK;
+}
+
 int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, const bntt_u128 *x, size_t n,
               const bntt_u128 *m, const bntt_u128 *mu)
 {
@@ -181,9 +216,21 @@ int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, cons
     std::lock_guard<std::mutex> lk(g_stage.mu);
     rc = stage_reserve(3 * bytes);
     if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    if (bytes <= kZeroCopyBytes) {
+        if ((rc = stage_host_reserve(3 * bytes))) return rc;
+        char *hb = static_cast<char *>(g_stage.hbuf), *hd = static_cast<char *>(g_stage.hdev);
+        std::memcpy(hb, a, bytes);
+        std::memcpy(hb + bytes, b, bytes);
+        rc = blas_dev(op, hd + 2 * bytes, hd, hd + bytes, x, n, m, mu, s);
+        if (rc) return rc;
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        std::memcpy(c, hb + 2 * bytes, bytes);
+        return BNTT_OK;
+    }
     char *base = static_cast<char *>(g_stage.buf);
     void *da = base, *db = base + bytes, *dc = base + 2 * bytes;
-    hipStream_t s = g_stage.stream;
     hipError_t e = hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
@@ -302,10 +349,23 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
     std::lock_guard<std::mutex> lk(g_stage.mu);
     rc = stage_reserve(3 * bytes + tmpb);
     if (rc) return rc;
+    hipStream_t s = g_stage.stream;
+    if (tmpb == 0 && bytes <= kZeroCopyBytes) {
+        // one pass: the kernel reads x and the table once and writes y once
+        if ((rc = stage_host_reserve(3 * bytes))) return rc;
+        char *hb = static_cast<char *>(g_stage.hbuf), *hd = static_cast<char *>(g_stage.hdev);
+        std::memcpy(hb, X, bytes);
+        std::memcpy(hb + bytes, tw, bytes);
+        rc = bntt_ntt_fwd_dev(hd + 2 * bytes, hd, hd + bytes, log2n, 1, m, mu, nullptr, s);
+        if (rc) return rc;
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        std::memcpy(Y, hb + 2 * bytes, bytes);
+        return BNTT_OK;
+    }
     char *base = static_cast<char *>(g_stage.buf);
     void *dx = base, *dtw = base + bytes, *dy = base + 2 * bytes;
     void *dtmp = tmpb ? base + 3 * bytes : nullptr;
-    hipStream_t s = g_stage.stream;
     hipError_t e = hipMemcpyAsync(dx, X, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dtw, tw, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
