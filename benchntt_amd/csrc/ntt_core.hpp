@@ -26,9 +26,17 @@ namespace bntt {
 
 typedef uint32_t v4u __attribute__((ext_vector_type(4)));
 
+// Device-memory accesses go through the global address space.  Pointers read
+// out of descriptors are generic to the compiler, and its flat loads count on
+// the LDS counter as well, so every wait for an LDS read would also wait for
+// the outstanding device-memory loads.  (Every such pointer here is device or
+// pinned host memory, never LDS or scratch.)
+typedef __attribute__((address_space(1))) const v4u *gv4u_cptr;
+typedef __attribute__((address_space(1))) v4u *gv4u_ptr;
+
 __device__ __forceinline__ U128 ld16(const U128 *p)
 {
-    const v4u t = *reinterpret_cast<const v4u *>(p);
+    const v4u t = *(gv4u_cptr)(reinterpret_cast<const v4u *>(p));
     U128 r;
     r.v[0] = t.x;
     r.v[1] = t.y;
@@ -44,7 +52,19 @@ __device__ __forceinline__ void st16(U128 *p, const U128 &a)
     t.y = a.v[1];
     t.z = a.v[2];
     t.w = a.v[3];
-    *reinterpret_cast<v4u *>(p) = t;
+    *(gv4u_ptr)(reinterpret_cast<v4u *>(p)) = t;
+}
+
+// A digit twiddle from a table in device memory (16-byte entries).
+__device__ __forceinline__ R31 ldr31(const R31 *p)
+{
+    const v4u t = *(gv4u_cptr)(reinterpret_cast<const v4u *>(p));
+    R31 r;
+    r.d[0] = t.x;
+    r.d[1] = t.y;
+    r.d[2] = t.z;
+    r.d[3] = t.w;
+    return r;
 }
 
 constexpr int kMaxThreads = 512;
@@ -187,7 +207,7 @@ __device__ __forceinline__ void fetch_twiddles(R31 (&t)[(1 << KP) - 1], const Tw
         } else {
             const R31 *tb = tv.gr31 + (1u << (S + r)) + (P << rp);
 #pragma unroll
-            for (int j = 0; j < (1 << rp); j++) t[(1 << rp) - 1 + j] = tb[j];
+            for (int j = 0; j < (1 << rp); j++) t[(1 << rp) - 1 + j] = ldr31(tb + j);
         }
     }
 }
@@ -413,14 +433,14 @@ __device__ __forceinline__ void stage_twiddles_r31(R31 *tws, const R31 *__restri
 {
     const int ntw = (int)(((1u << klds) - 1u) << wlog);  // stages [S, S + klds)
     if (S == 0) {
-        for (int e = threadIdx.x; e < ntw; e += blockDim.x) tws[tidx(e)] = twr[1 + e];
+        for (int e = threadIdx.x; e < ntw; e += blockDim.x) tws[tidx(e)] = ldr31(twr + 1 + e);
         return;
     }
     for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
         const uint32_t u = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
         const int r = 31 - __builtin_clz(u);
         const uint32_t off = (uint32_t)e - (((1u << r) - 1u) << wlog);
-        tws[tidx(e)] = twr[(1u << (S + r)) + (P0start << r) + off];
+        tws[tidx(e)] = ldr31(twr + (1u << (S + r)) + (P0start << r) + off);
     }
 }
 

@@ -271,7 +271,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         // a large twiddle, or a value above q (decided exactly in the rare
         // workgroups where some value is near q): the literal formulas.  The
         // vote is also the barrier after the tile writes.
-        const bool twb = *d0.twflag != 0;  // one flag for the whole item
+        const bool twb = *(const __attribute__((address_space(1))) uint32_t *)d0.twflag != 0;  // one flag per item
         if (__syncthreads_or(big || twb) && (twb || tile_above_q(lds, nvalid << k, k, md.m))) return false;
     } else {
         // Montgomery: no literal fallback, inputs must be canonical; slow
@@ -336,7 +336,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                     // position ti of c takes network output brv(ti): contiguous
                     // stores, factor from the table permuted to positions
                     const U128 y = lds[lidx((g << k) + (int)bitrev(ti, (uint32_t)k), k)];
-                    st16(s_y[g] + ti, montmul_r31<SPARSE>(y, d0.scale[((size_t)1 << k) + ti], md.mq));
+                    st16(s_y[g] + ti, montmul_r31<SPARSE>(y, ldr31(d0.scale + ((size_t)1 << k) + ti), md.mq));
                 } else {
                     const U128 y = lds[lidx(e, k)];
                     st16(s_y[g] + ti, canon ? canonical(y, md.mq) : y);
@@ -346,7 +346,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
             if ((e >> k) < nvalid) {
                 const uint32_t idx = ((uint32_t)it.first << k) + (uint32_t)e;
                 if (scatter)
-                    st16(dst + bitrev(idx, (uint32_t)L), montmul_r31<SPARSE>(lds[lidx(e, k)], d0.scale[idx], md.mq));
+                    st16(dst + bitrev(idx, (uint32_t)L), montmul_r31<SPARSE>(lds[lidx(e, k)], ldr31(d0.scale + idx), md.mq));
                 else if (canon)
                     st16(dst + idx, canonical(lds[lidx(e, k)], md.mq));
                 else
