@@ -27,17 +27,25 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     // HBM -> LDS.  Single-pass (k == L): groups are whole transforms and the
     // block's data is one contiguous range.  Otherwise element T_idx of group g
     // sits at transform*N + T_idx*2^mlog + M; consecutive threads take
-    // consecutive g (consecutive addresses).
+    // consecutive g (consecutive addresses).  All E loads of a thread are
+    // issued before the first is used (an element past the block's groups
+    // reads the block's first group instead, and is not kept), so they are in
+    // flight together rather than one at a time.
+    U128 a[E];
     if (mlog == 0) {
         // the workgroup's G transforms are one contiguous range
         const U128 *xb = x + (gamma0 << L);
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int e = threadIdx.x + j * T;
+            a[j] = ld16(xb + ((e >> k) < nvalid ? e : 0));
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * T;
             if ((e >> k) < nvalid) {
-                const U128 a = ld16(xb + e);
-                big |= near_q(a, md.m);
-                lds[lidx(e, k)] = a;
+                big |= near_q(a[j], md.m);
+                lds[lidx(e, k)] = a[j];
             }
         }
     } else {
@@ -45,13 +53,18 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
         for (int j = 0; j < E; j++) {
             const int e = threadIdx.x + j * T;
             const int g = e & (G - 1), ti = e >> Glog;
+            const size_t gamma = gamma0 + (size_t)(g < nvalid ? g : 0);
+            const size_t tr = gamma >> mlog;
+            const size_t M = gamma & (((size_t)1 << mlog) - 1);
+            a[j] = ld16(x + (tr << L) + ((size_t)ti << mlog) + M);
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * T;
+            const int g = e & (G - 1), ti = e >> Glog;
             if (g < nvalid) {
-                const size_t gamma = gamma0 + (size_t)g;
-                const size_t tr = gamma >> mlog;
-                const size_t M = gamma & (((size_t)1 << mlog) - 1);
-                const U128 a = ld16(x + (tr << L) + ((size_t)ti << mlog) + M);
-                big |= near_q(a, md.m);
-                lds[lidx((g << k) + ti, k)] = a;
+                big |= near_q(a[j], md.m);
+                lds[lidx((g << k) + ti, k)] = a[j];
             }
         }
     }

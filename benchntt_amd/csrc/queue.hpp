@@ -181,6 +181,109 @@ __device__ __forceinline__ uint32_t bitrev(uint32_t i, uint32_t log2n)
     return log2n ? (__brev(i) >> (32u - log2n)) : 0u;
 }
 
+// HBM -> LDS for one transform item, E values per thread.  All of a thread's
+// loads are issued before the first is used (an element past the item's
+// groups reads its first group instead and is not kept), so they are in
+// flight together rather than one at a time.  Returns whether a kept value is
+// near q (reference arithmetic).
+template <int E, int KIND>
+__device__ __forceinline__ bool load_tile(U128 *lds, const U128 *const *s_x, const U128 *const *s_x2,
+                                          const WorkItem &it, const DevDesc &d0, const U128 *src, const Mod &md)
+{
+    constexpr bool MONT = qk_mont(KIND), SPARSE = KIND == QK_MONT_S;
+    const int L = it.L, k = it.k, Glog = it.Glog, mlog = L - k, nvalid = (int)it.nvalid;
+    const int G = 1 << Glog;
+    const bool gather = MONT && (it.flags & (IT_GATHER_MUL | IT_GATHER)) && it.pass == 0;
+    const bool gmul = (it.flags & IT_GATHER_MUL) != 0;
+    const bool contig = !MONT && (it.flags & IT_CONTIG);
+    const U128 one{{1u, 0u, 0u, 0u}};
+    U128 a[E], b[E];
+    bool big = false;
+    if (contig) {
+        // transforms back to back: one contiguous range
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            a[j] = ld16(d0.x + ((e >> k) < nvalid ? e : 0));
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            if ((e >> k) < nvalid) {
+                big |= near_q(a[j], md.m);
+                lds[lidx(e, k)] = a[j];
+            }
+        }
+    } else if (mlog == 0) {
+        // whole transforms, one per group, each with its own buffers
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            const int g = e >> k, ti = e & ((1 << k) - 1);
+            const int gv = g < nvalid ? g : 0;
+            a[j] = ld16(s_x[gv] + ti);
+            if (gather && gmul) b[j] = ld16(s_x2[gv] + ti);
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            const int g = e >> k, ti = e & ((1 << k) - 1);
+            if (g < nvalid) {
+                if (gather) {
+                    // network input brv(j) is A[j] * B[j]: contiguous loads,
+                    // bit-reversed placement in LDS
+                    lds[lidx((g << k) + (int)bitrev((uint32_t)ti, (uint32_t)k), k)] =
+                        gather_product<SPARSE>(a[j], gmul ? b[j] : one, md.mq);
+                } else {
+                    big |= near_q(a[j], md.m);
+                    lds[lidx(e, k)] = a[j];
+                }
+            }
+        }
+    } else if (gather) {
+        // group g's inputs (ti << mlog) + M, bit-reversed, are the contiguous
+        // block brv(M) << k .. + 2^k of A and B (in the order brv(ti)): load
+        // the block, place each value at its ti
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            const int g = e >> k, jj = e & ((1 << k) - 1);
+            const uint32_t gv = (uint32_t)(g < nvalid ? g : 0);
+            const uint32_t blk = bitrev((uint32_t)it.first + gv, (uint32_t)mlog) << k;
+            a[j] = ld16(d0.x + blk + jj);
+            if (gmul) b[j] = ld16(d0.x2 + blk + jj);
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            const int g = e >> k, jj = e & ((1 << k) - 1);
+            if (g < nvalid)
+                lds[lidx((g << k) + (int)bitrev((uint32_t)jj, (uint32_t)k), k)] =
+                    gather_product<SPARSE>(a[j], gmul ? b[j] : one, md.mq);
+        }
+    } else {
+        // blocks of one transform: element ti of group g at (ti << mlog) + M
+        // (indices within one transform, at most 2^24 points, fit 32 bits)
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            const int g = e & (G - 1), ti = e >> Glog;
+            const uint32_t gv = (uint32_t)(g < nvalid ? g : 0);
+            a[j] = ld16(src + ((uint32_t)ti << mlog) + (uint32_t)it.first + gv);
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = threadIdx.x + j * kQueueThreads;
+            const int g = e & (G - 1), ti = e >> Glog;
+            if (g < nvalid) {
+                big |= near_q(a[j], md.m);
+                lds[lidx((g << k) + ti, k)] = a[j];
+            }
+        }
+    }
+    return big;
+}
+
 // One transform item: load, network, store.  Returns false when a fast
 // launch leaves the item to the literal formulas (nothing written).
 template <int KIND>
@@ -223,48 +326,16 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
         __syncthreads();
     }
     // HBM -> LDS
-    for (int j = 0; j < E; j++) {
-        const int e = threadIdx.x + j * kQueueThreads;
-        if (contig) {
-            if ((e >> k) < nvalid) {
-                const U128 a = ld16(d0.x + e);
-                big |= near_q(a, md.m);
-                lds[lidx(e, k)] = a;
-            }
-        } else if (mlog == 0) {
-            const int g = e >> k, ti = e & ((1 << k) - 1);
-            if (g < nvalid) {
-                if (gather) {
-                    // network input brv(j) is A[j] * B[j]: contiguous loads,
-                    // bit-reversed placement in LDS
-                    const U128 a = gather_product<SPARSE>(ld16(s_x[g] + ti), gmul ? ld16(s_x2[g] + ti) : one, md.mq);
-                    lds[lidx((g << k) + (int)bitrev((uint32_t)ti, (uint32_t)k), k)] = a;
-                } else {
-                    const U128 a = ld16(s_x[g] + ti);
-                    big |= near_q(a, md.m);
-                    lds[lidx(e, k)] = a;
-                }
-            }
-        } else if (gather) {
-            // group g's inputs (ti << mlog) + M, bit-reversed, are the
-            // contiguous block brv(M) << k .. + 2^k of A and B (in the order
-            // brv(ti)): load the block, place each value at its ti
-            const int g = e >> k, j = e & ((1 << k) - 1);
-            if (g < nvalid) {
-                const uint32_t blk = bitrev((uint32_t)it.first + (uint32_t)g, (uint32_t)mlog) << k;
-                const U128 a = gather_product<SPARSE>(ld16(d0.x + blk + j), gmul ? ld16(d0.x2 + blk + j) : one, md.mq);
-                lds[lidx((g << k) + (int)bitrev((uint32_t)j, (uint32_t)k), k)] = a;
-            }
-        } else {
-            const int g = e & (G - 1), ti = e >> Glog;
-            if (g < nvalid) {
-                // indices within one transform (at most 2^24 points) fit 32 bits
-                const uint32_t idx = ((uint32_t)ti << mlog) + (uint32_t)it.first + (uint32_t)g;
-                const U128 a = ld16(src + idx);
-                big |= near_q(a, md.m);
-                lds[lidx((g << k) + ti, k)] = a;
-            }
-        }
+    switch (elog) {
+    case 1:
+        big = load_tile<2, KIND>(lds, s_x, s_x2, it, d0, src, md);
+        break;
+    case 2:
+        big = load_tile<4, KIND>(lds, s_x, s_x2, it, d0, src, md);
+        break;
+    default:
+        big = load_tile<8, KIND>(lds, s_x, s_x2, it, d0, src, md);
+        break;
     }
     const uint32_t mbase = (uint32_t)it.first;
     if constexpr (KIND == QK_FAST) {
