@@ -404,24 +404,50 @@ __device__ __forceinline__ bool stage_twiddles(R31 *tws, const U128 *__restrict_
     bool big = false;
     const int ntw = (int)(((1u << k) - 1u) << wlog);
     const int nstaged = (int)(((1u << (klds < k ? klds : k)) - 1u) << wlog);  // stages r < klds come first
+    // four loads of a thread in flight at once (an entry past the tables reads
+    // entry 1 instead and is not kept)
+    constexpr int U = 4;
+    const int T = blockDim.x;
     if (S == 0) {
         // the first stages of a transform (one prefix): the staged tables are
         // the table itself from entry 1 on
-        for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
-            const U128 a = ld16(tw + 1 + e);
-            big |= tw_big(a);
-            if (e < nstaged) tws[tidx(e)] = to_r31(a);
+        for (int e0 = threadIdx.x; e0 < ntw; e0 += U * T) {
+            U128 a[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * T;
+                a[u] = ld16(tw + 1 + (e < ntw ? e : 0));
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int e = e0 + u * T;
+                if (e < ntw) {
+                    big |= tw_big(a[u]);
+                    if (e < nstaged) tws[tidx(e)] = to_r31(a[u]);
+                }
+            }
         }
         return big;
     }
-    for (int e = threadIdx.x; e < ntw; e += blockDim.x) {
-        // e = W*(2^r - 1) + rel*2^r + j  ->  stage r, offset within the stage table
-        const uint32_t u = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
-        const int r = 31 - __builtin_clz(u);
-        const uint32_t off = (uint32_t)e - (((1u << r) - 1u) << wlog);
-        const U128 a = ld16(tw + (1u << (S + r)) + (P0start << r) + off);
-        big |= tw_big(a);
-        if (e < nstaged) tws[tidx(e)] = to_r31(a);
+    for (int e0 = threadIdx.x; e0 < ntw; e0 += U * T) {
+        U128 a[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            // e = W*(2^r - 1) + rel*2^r + j  ->  stage r, offset within the stage table
+            const int e = e0 + u * T < ntw ? e0 + u * T : 0;
+            const uint32_t v = ((uint32_t)e >> wlog) + 1u;  // in [2^r, 2^(r+1))
+            const int r = 31 - __builtin_clz(v);
+            const uint32_t off = (uint32_t)e - (((1u << r) - 1u) << wlog);
+            a[u] = ld16(tw + (1u << (S + r)) + (P0start << r) + off);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int e = e0 + u * T;
+            if (e < ntw) {
+                big |= tw_big(a[u]);
+                if (e < nstaged) tws[tidx(e)] = to_r31(a[u]);
+            }
+        }
     }
     return big;
 }
