@@ -308,13 +308,14 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     return out
 
 
-def serving(torch, np, n_req: int = 4096, cpu: bool = True) -> dict:
+def serving(torch, np, n_req: int = 16384, cpu: bool = True) -> dict:
     """Negacyclic polynomial-multiply requests mod q_116 (forward NTT x2,
     pointwise product, inverse NTT) with N uniform on 2^10..2^16, served with
     continuous batching under a 2 GiB HBM budget on 4 streams.  First a closed
     loop (every request arrived at t = 0) gives the saturation rate; then
     seeded Poisson arrivals at fractions of it give p50/p99 latency (arrival to
-    completion seen by the host) and the achieved rate.  Also: the same
+    completion seen by the host) and the achieved rate.  16384 requests per
+    trace, so that p99 rests on the 164 slowest.  Also: the same
     requests one at a time with no batching.  Operands: seeded uniform
     residues from the library's device generator (bntt_uniform_dev)."""
     from benchntt_amd import config as C
