@@ -149,9 +149,8 @@ struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
     bntt::QueueCtl *ctl = nullptr;
-    WorkItem *items_d = nullptr;
-    DevDesc *descs_d = nullptr;
-    char *stage_h = nullptr;  // pinned staging: items | descs
+    char *batch_d = nullptr;  // a batch's items, then its descriptors (one upload)
+    char *stage_h = nullptr;  // pinned staging of the same layout
     size_t cap_items = 0, cap_descs = 0;
     bool busy = false;
     std::vector<size_t> reqs;                          // request indices of the batch
@@ -204,8 +203,7 @@ void destroy_server(bntt_pm_server *s)
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
         if (sl.done) (void)hipEventDestroy(sl.done);
         (void)hipFree(sl.ctl);
-        (void)hipFree(sl.items_d);
-        (void)hipFree(sl.descs_d);
+        (void)hipFree(sl.batch_d);
         if (sl.stage_h) (void)hipHostFree(sl.stage_h);
     }
     (void)hipFree(s->arena_d);
@@ -273,8 +271,7 @@ int make_slot(Slot &sl, size_t cap_items, size_t cap_descs)
     sl.cap_items = cap_items;
     sl.cap_descs = cap_descs;
     if (hipMalloc(&sl.ctl, sizeof(bntt::QueueCtl)) != hipSuccess ||
-        hipMalloc(&sl.items_d, cap_items * sizeof(WorkItem)) != hipSuccess ||
-        hipMalloc(&sl.descs_d, cap_descs * sizeof(DevDesc)) != hipSuccess ||
+        hipMalloc(&sl.batch_d, cap_items * sizeof(WorkItem) + cap_descs * sizeof(DevDesc)) != hipSuccess ||
         hipHostMalloc(&sl.stage_h, cap_items * sizeof(WorkItem) + cap_descs * sizeof(DevDesc),
                       hipHostMallocDefault) != hipSuccess)
         return bntt_set_error(BNTT_ENOMEM, "allocating serving buffers failed");
@@ -400,8 +397,12 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
     for (auto &v : lv) nitems += v.size();
     if (nitems > sl.cap_items || descs.size() > sl.cap_descs)
         return bntt_set_error(BNTT_EINVAL, "internal: batch exceeds the slot capacity");
+    // descriptors follow the items directly (WorkItem is 32 bytes, so they
+    // stay aligned), and one copy uploads both
+    static_assert(sizeof(WorkItem) % alignof(DevDesc) == 0, "descriptor alignment");
+    const size_t desc_off = nitems * sizeof(WorkItem);
     WorkItem *it_h = reinterpret_cast<WorkItem *>(sl.stage_h);
-    DevDesc *de_h = reinterpret_cast<DevDesc *>(sl.stage_h + sl.cap_items * sizeof(WorkItem));
+    DevDesc *de_h = reinterpret_cast<DevDesc *>(sl.stage_h + desc_off);
     std::vector<size_t> off(kLevels);
     size_t pos = 0;
     for (int l = 0; l < kLevels; l++) {
@@ -410,9 +411,8 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
         pos += lv[l].size();
     }
     std::memcpy(de_h, descs.data(), descs.size() * sizeof(DevDesc));
-    hipError_t e = hipMemcpyAsync(sl.items_d, it_h, nitems * sizeof(WorkItem), hipMemcpyHostToDevice, sl.stream);
-    if (e == hipSuccess && !descs.empty())
-        e = hipMemcpyAsync(sl.descs_d, de_h, descs.size() * sizeof(DevDesc), hipMemcpyHostToDevice, sl.stream);
+    hipError_t e = hipMemcpyAsync(sl.batch_d, sl.stage_h, desc_off + descs.size() * sizeof(DevDesc),
+                                  hipMemcpyHostToDevice, sl.stream);
     if (e != hipSuccess) return bntt_set_hip_error(e, "hipMemcpyAsync of a batch");
     for (int l = 0; l < kLevels; l++) {
         if (lv[l].empty()) continue;
@@ -425,8 +425,8 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
         bntt::TwArea area;
         const size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), s->tw_budget, &area);
         const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)(lds > bntt::kQueueLds3 ? s->grid_cap2 : s->grid_cap));
-        launch_serving(s, dim3(grid), lds, sl.stream, static_cast<const WorkItem *>(sl.items_d + off[l]),
-                       (uint32_t)lv[l].size(), static_cast<const DevDesc *>(sl.descs_d),
+        launch_serving(s, dim3(grid), lds, sl.stream, reinterpret_cast<const WorkItem *>(sl.batch_d) + off[l],
+                       (uint32_t)lv[l].size(), reinterpret_cast<const DevDesc *>(sl.batch_d + desc_off),
                        static_cast<const bntt::DevMul *>(nullptr), sl.ctl, area, s->md,
                        bntt::PuntList{nullptr, nullptr});
         e = hipGetLastError();
