@@ -8,8 +8,11 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 
@@ -101,11 +104,27 @@ int ensure_init()
 
 bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
 
-int check_vec(const void *c, const void *a, const void *b, const bntt_u128 *m)
+// Largest element count whose three arrays fit a size_t byte count.
+constexpr size_t kMaxElems = SIZE_MAX / (3 * sizeof(bntt_u128));
+
+// [p, p + bytes) and [q, q + bytes) share a byte but do not start together.
+bool partial_overlap(const void *p, const void *q, size_t bytes)
+{
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p), b = reinterpret_cast<uintptr_t>(q);
+    return a != b && a < b + bytes && b < a + bytes;
+}
+
+int check_vec(const void *c, const void *a, const void *b, size_t n, const bntt_u128 *m)
 {
     if (!c || !a || !b || !m) return fail(BNTT_EINVAL, "null pointer argument");
     if (!aligned16(c) || !aligned16(a) || !aligned16(b))
         return fail(BNTT_EINVAL, "device arrays must be 16-byte aligned");
+    if (n > kMaxElems) return fail(BNTT_EINVAL, "n = %zu is too large", n);
+    // each element is read and written by one thread: c may be a or b, but
+    // a shifted overlap would let one element's write race another's read
+    const size_t bytes = n * sizeof(bntt_u128);
+    if (partial_overlap(c, a, bytes) || partial_overlap(c, b, bytes))
+        return fail(BNTT_EINVAL, "c must equal a (or b) or not overlap it");
     return BNTT_OK;
 }
 
@@ -115,7 +134,7 @@ int blas_dev(int op, void *c, const void *a, const void *b, const bntt_u128 *x, 
     int rc = ensure_init();
     if (rc) return rc;
     if (n == 0) return BNTT_OK;
-    rc = check_vec(c, a, b, m);
+    rc = check_vec(c, a, b, n, m);
     if (rc) return rc;
     if ((op == bntt::OP_VVMUL || op == bntt::OP_AXPY) && !mu) return fail(BNTT_EINVAL, "mu is required");
     if (op == bntt::OP_AXPY && !x) return fail(BNTT_EINVAL, "axpy scalar is required");
@@ -127,11 +146,12 @@ int blas_dev(int op, void *c, const void *a, const void *b, const bntt_u128 *x, 
     return BNTT_OK;
 }
 
-// Library-owned staging for the host-pointer entry points: one stream, one
-// grow-only device arena and one grow-only pinned host buffer per process
-// (current device at first use).  Small calls run zero-copy: the caller's
-// arrays are copied into the pinned buffer, the kernel reads and writes it
-// over PCIe, and the result is copied out, with no DMA transfer.
+// Library-owned staging for the host-pointer entry points: per device, one
+// stream, one grow-only device arena and one grow-only pinned host buffer,
+// kept until the process ends; calls on one device take turns.  Small calls
+// run zero-copy: the caller's arrays are copied into the pinned buffer, the
+// kernel reads and writes it over PCIe, and the result is copied out, with
+// no DMA transfer.
 struct Staging {
     std::mutex mu;
     hipStream_t stream = nullptr;
@@ -140,68 +160,70 @@ struct Staging {
     void *hbuf = nullptr;  // pinned host memory
     void *hdev = nullptr;  // its address for kernels
     size_t hcap = 0;
-    int dev = -1;
 };
-Staging g_stage;
+std::mutex g_stages_mu;
+std::map<int, std::unique_ptr<Staging>> g_stages;
+
+// The current device's staging (created empty at first use).
+int stage_for_current(Staging **out)
+{
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_stages_mu);
+    std::unique_ptr<Staging> &st = g_stages[dev];
+    if (!st) st.reset(new Staging());
+    *out = st.get();
+    return BNTT_OK;
+}
 
 // Largest arrays (bytes each) that the host-pointer entry points move zero-copy.
 constexpr size_t kZeroCopyBytes = (size_t)4096 * sizeof(bntt_u128);
 
-int stage_reserve(size_t bytes)
+// The device arena, at least `bytes` (caller holds st.mu, current device = st's).
+int stage_reserve(Staging &stg, size_t bytes)
 {
-    int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
-    if (g_stage.dev != dev) {
-        // different device than the arena's: start a new arena (the old one
-        // stays owned by its device until the process ends)
-        g_stage.stream = nullptr;
-        g_stage.buf = nullptr;
-        g_stage.cap = 0;
-        g_stage.hbuf = g_stage.hdev = nullptr;
-        g_stage.hcap = 0;
-        g_stage.dev = dev;
-    }
-    if (!g_stage.stream) {
-        e = hipStreamCreate(&g_stage.stream);
+    hipError_t e;
+    if (!stg.stream) {
+        e = hipStreamCreate(&stg.stream);
         if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
     }
-    if (g_stage.cap < bytes) {
-        if (g_stage.buf) {
-            e = hipStreamSynchronize(g_stage.stream);
+    if (stg.cap < bytes) {
+        if (stg.buf) {
+            e = hipStreamSynchronize(stg.stream);
             if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-            (void)hipFree(g_stage.buf);
-            g_stage.buf = nullptr;
-            g_stage.cap = 0;
+            (void)hipFree(stg.buf);
+            stg.buf = nullptr;
+            stg.cap = 0;
         }
-        e = hipMalloc(&g_stage.buf, bytes);
+        e = hipMalloc(&stg.buf, bytes);
         if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
-        g_stage.cap = bytes;
+        stg.cap = bytes;
     }
     return BNTT_OK;
 }
 
-// The pinned host buffer, at least `bytes` (after stage_reserve on this device).
-int stage_host_reserve(size_t bytes)
+// The pinned host buffer, at least `bytes` (after stage_reserve).
+int stage_host_reserve(Staging &stg, size_t bytes)
 {
-    if (g_stage.hcap >= bytes) return BNTT_OK;
+    if (stg.hcap >= bytes) return BNTT_OK;
     hipError_t e;
-    if (g_stage.hbuf) {
-        e = hipStreamSynchronize(g_stage.stream);
+    if (stg.hbuf) {
+        e = hipStreamSynchronize(stg.stream);
         if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-        (void)hipHostFree(g_stage.hbuf);
-        g_stage.hbuf = g_stage.hdev = nullptr;
-        g_stage.hcap = 0;
+        (void)hipHostFree(stg.hbuf);
+        stg.hbuf = stg.hdev = nullptr;
+        stg.hcap = 0;
     }
-    e = hipHostMalloc(&g_stage.hbuf, bytes, hipHostMallocDefault);
+    e = hipHostMalloc(&stg.hbuf, bytes, hipHostMallocDefault);
     if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipHostMalloc(%zu): %s", bytes, hipGetErrorString(e));
-    e = hipHostGetDevicePointer(&g_stage.hdev, g_stage.hbuf, 0);
+    e = hipHostGetDevicePointer(&stg.hdev, stg.hbuf, 0);
     if (e != hipSuccess) {
-        (void)hipHostFree(g_stage.hbuf);
-        g_stage.hbuf = nullptr;
+        (void)hipHostFree(stg.hbuf);
+        stg.hbuf = nullptr;
         return hip_fail(e, "hipHostGetDevicePointer");
     }
-    g_stage.hcap = bytes;
+    stg.hcap = bytes;
     return BNTT_OK;
 }
 
@@ -212,14 +234,18 @@ int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, cons
     if (rc) return rc;
     if (n == 0) return BNTT_OK;
     if (!c || !a || !b || !m) return fail(BNTT_EINVAL, "null pointer argument");
+    if (n > kMaxElems) return fail(BNTT_EINVAL, "n = %zu is too large", n);
     const size_t bytes = n * sizeof(bntt_u128);
-    std::lock_guard<std::mutex> lk(g_stage.mu);
-    rc = stage_reserve(3 * bytes);
+    Staging *st = nullptr;
+    if ((rc = stage_for_current(&st))) return rc;
+    Staging &stg = *st;
+    std::lock_guard<std::mutex> lk(stg.mu);
+    rc = stage_reserve(stg, 3 * bytes);
     if (rc) return rc;
-    hipStream_t s = g_stage.stream;
+    hipStream_t s = stg.stream;
     if (bytes <= kZeroCopyBytes) {
-        if ((rc = stage_host_reserve(3 * bytes))) return rc;
-        char *hb = static_cast<char *>(g_stage.hbuf), *hd = static_cast<char *>(g_stage.hdev);
+        if ((rc = stage_host_reserve(stg, 3 * bytes))) return rc;
+        char *hb = static_cast<char *>(stg.hbuf), *hd = static_cast<char *>(stg.hdev);
         std::memcpy(hb, a, bytes);
         std::memcpy(hb + bytes, b, bytes);
         rc = blas_dev(op, hd + 2 * bytes, hd, hd + bytes, x, n, m, mu, s);
@@ -229,7 +255,7 @@ int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, cons
         std::memcpy(c, hb + 2 * bytes, bytes);
         return BNTT_OK;
     }
-    char *base = static_cast<char *>(g_stage.buf);
+    char *base = static_cast<char *>(stg.buf);
     void *da = base, *db = base + bytes, *dc = base + 2 * bytes;
     hipError_t e = hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, s);
@@ -284,6 +310,7 @@ int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, siz
 size_t bntt_ntt_tmp_bytes(int log2n, size_t count)
 {
     if (log2n <= bntt::kLdsMaxLog2 || log2n > bntt::kMaxLog2) return 0;
+    if (count > (SIZE_MAX / sizeof(bntt_u128)) >> log2n) return SIZE_MAX;  // no such buffer exists
     return (count << log2n) * sizeof(bntt_u128);
 }
 
@@ -298,6 +325,7 @@ int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t c
     if (!y || !x || !tw || !m || !mu) return fail(BNTT_EINVAL, "null pointer argument");
     if (!aligned16(y) || !aligned16(x) || !aligned16(tw) || (tmp && !aligned16(tmp)))
         return fail(BNTT_EINVAL, "device arrays must be 16-byte aligned");
+    if (count > (SIZE_MAX / sizeof(bntt_u128)) >> log2n) return fail(BNTT_EINVAL, "count = %zu is too large", count);
     const size_t n = (size_t)1 << log2n;
     if (log2n > bntt::kLdsMaxLog2) {
         if (!tmp) return fail(BNTT_EINVAL, "log2n = %d needs a tmp buffer of bntt_ntt_tmp_bytes()", log2n);
@@ -346,14 +374,17 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
     const size_t n = (size_t)1 << log2n;
     const size_t bytes = n * sizeof(bntt_u128);
     const size_t tmpb = bntt_ntt_tmp_bytes(log2n, 1);
-    std::lock_guard<std::mutex> lk(g_stage.mu);
-    rc = stage_reserve(3 * bytes + tmpb);
+    Staging *st = nullptr;
+    if ((rc = stage_for_current(&st))) return rc;
+    Staging &stg = *st;
+    std::lock_guard<std::mutex> lk(stg.mu);
+    rc = stage_reserve(stg, 3 * bytes + tmpb);
     if (rc) return rc;
-    hipStream_t s = g_stage.stream;
+    hipStream_t s = stg.stream;
     if (tmpb == 0 && bytes <= kZeroCopyBytes) {
         // one pass: the kernel reads x and the table once and writes y once
-        if ((rc = stage_host_reserve(3 * bytes))) return rc;
-        char *hb = static_cast<char *>(g_stage.hbuf), *hd = static_cast<char *>(g_stage.hdev);
+        if ((rc = stage_host_reserve(stg, 3 * bytes))) return rc;
+        char *hb = static_cast<char *>(stg.hbuf), *hd = static_cast<char *>(stg.hdev);
         std::memcpy(hb, X, bytes);
         std::memcpy(hb + bytes, tw, bytes);
         rc = bntt_ntt_fwd_dev(hd + 2 * bytes, hd, hd + bytes, log2n, 1, m, mu, nullptr, s);
@@ -363,7 +394,7 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
         std::memcpy(Y, hb + 2 * bytes, bytes);
         return BNTT_OK;
     }
-    char *base = static_cast<char *>(g_stage.buf);
+    char *base = static_cast<char *>(stg.buf);
     void *dx = base, *dtw = base + bytes, *dy = base + 2 * bytes;
     void *dtmp = tmpb ? base + 3 * bytes : nullptr;
     hipError_t e = hipMemcpyAsync(dx, X, bytes, hipMemcpyHostToDevice, s);

@@ -5,6 +5,8 @@
 // (ntt_scalar.c:49-199) as host functions.
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+
 #include "../../include/bntt.h"
 #include "hostmont.hpp"
 #include "internal.hpp"
@@ -160,6 +162,7 @@ int via_device(bntt_u128 *host, size_t n, F fill)
 {
     if (!host && n) return bntt_set_error(BNTT_EINVAL, "null output");
     if (!n) return BNTT_OK;
+    if (n > SIZE_MAX / sizeof(bntt_u128)) return bntt_set_error(BNTT_EINVAL, "n is too large");
     void *d = nullptr;
     if (hipMalloc(&d, n * sizeof(bntt_u128)) != hipSuccess) return bntt_set_error(BNTT_ENOMEM, "hipMalloc failed");
     int rc = fill(d);
@@ -231,6 +234,13 @@ int bntt_hilo_to_u128_dev(void *out, const void *hilo, size_t n, void *stream)
     if (rc) return rc;
     if (!n) return BNTT_OK;
     if ((rc = check_dev_ptr(out, "out")) || (rc = check_dev_ptr(hilo, "hilo"))) return rc;
+    // in place is fine (each element is read and written by one thread); a
+    // shifted overlap would race
+    const uintptr_t o = reinterpret_cast<uintptr_t>(out), h = reinterpret_cast<uintptr_t>(hilo);
+    if (n > SIZE_MAX / sizeof(bntt_u128)) return bntt_set_error(BNTT_EINVAL, "n is too large");
+    const size_t bytes = n * sizeof(bntt_u128);
+    if (o != h && o < h + bytes && h < o + bytes)
+        return bntt_set_error(BNTT_EINVAL, "out must equal the input or not overlap it");
     hipLaunchKernelGGL(swap_words_kernel, dim3(grid_for(n)), dim3(kIoThreads), 0, static_cast<hipStream_t>(stream),
                        static_cast<ulonglong2 *>(out), static_cast<const ulonglong2 *>(hilo), n);
     return launched("word-order conversion launch");

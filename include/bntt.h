@@ -66,6 +66,9 @@ const char *bntt_last_error(void);
 
 /* ------------------------------------------------------------ device pointers */
 
+/* Vector ops: c may be a or b (in place) but may not overlap them otherwise
+ * (BNTT_EINVAL); n above SIZE_MAX / 48 is BNTT_EINVAL. */
+
 /* c[i] = a[i] + b[i] mod m, i < n (ntt_scalar.c:49-79 semantics). */
 int bntt_vvadd_dev(void *c, const void *a, const void *b, size_t n, const bntt_u128 *m, void *stream);
 /* c[i] = a[i] - b[i] mod m. */
@@ -87,6 +90,8 @@ int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, siz
  * one-transform path. */
 int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t count, const bntt_u128 *m,
                      const bntt_u128 *mu, void *tmp, void *stream);
+/* Bytes of tmp for bntt_ntt_fwd_dev: 0 when log2n <= 12 (no tmp needed),
+ * SIZE_MAX when count * 2^log2n * 16 does not fit a size_t. */
 size_t bntt_ntt_tmp_bytes(int log2n, size_t count);
 
 /* ------------------------------------------------------------ uneven batches */
@@ -249,8 +254,8 @@ bntt_u128 bntt_mulmod128(bntt_u128 a, bntt_u128 b, bntt_u128 m, bntt_u128 mu);
 /* Word order of the reference's fixture files and AVX code: residue i is
  * hilo[2i] = high word, hilo[2i+1] = low word (config.c:6-197 reads "hi lo"
  * pairs into this layout; ntt_avx512.c:386-528 takes it).  n residues; out may
- * equal the input.  Host versions convert on the host; _dev versions run on
- * `stream`. */
+ * equal the input (the _dev versions reject any other overlap).  Host versions
+ * convert on the host; _dev versions run on `stream`. */
 int bntt_hilo_to_u128(bntt_u128 *out, const uint64_t *hilo, size_t n);
 int bntt_u128_to_hilo(uint64_t *hilo, const bntt_u128 *in, size_t n);
 int bntt_hilo_to_u128_dev(void *out, const void *hilo, size_t n, void *stream);

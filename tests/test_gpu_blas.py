@@ -139,6 +139,30 @@ def test_errors(gpu):
         bn.vector_add(mis, mis, mis, QB)
 
 
+@pytest.mark.parametrize("op", OPS)
+def test_in_place_and_shifted_overlap(gpu, orc, op):
+    """c may be a or b (the result replaces that operand); c shifted by one
+    residue against an operand is rejected before any launch."""
+    import benchntt_amd as bn
+
+    n = 1000
+    a, b = uniform(n, QB, 91), uniform(n, QB, 92)
+    for target in ("a", "b"):
+        da, db = to_dev(gpu, a), to_dev(gpu, b)
+        dc = da if target == "a" else db
+        call = {"vvadd": lambda: bn.vector_add(dc, da, db, QB), "vvsub": lambda: bn.vector_sub(dc, da, db, QB),
+                "vvmul": lambda: bn.vector_mul(dc, da, db, QB, MUB),
+                "axpy": lambda: bn.axpy(dc, da, db, S, QB, MUB)}[op]
+        call()
+        gpu.cuda.synchronize()
+        assert np.array_equal(from_dev(dc), want(orc, op, a, b)), target
+    x = to_dev(gpu, uniform(n + 1, QB, 93))
+    with pytest.raises(bn.BnttError):
+        bn.vector_add(x[1:], x[:n], x[:n], QB)
+    with pytest.raises(bn.BnttError):
+        bn.axpy(x[:n], to_dev(gpu, a), x[1:], S, QB, MUB)
+
+
 def test_host_entry_points_from_many_threads(gpu, orc):
     """The host-pointer entry points are thread-safe: 8 threads call axpy,
     vvmul and ntt on host arrays at once (shared staging buffers), each result

@@ -80,6 +80,11 @@ def test_word_order_kernels(gpu):
         assert gpu.equal(fx.hilo_to_u128(hl), d)
         fx.hilo_to_u128(d, out=d)  # in place
         assert np.array_equal(from_dev(d), a[:, ::-1])
+    from benchntt_amd import BnttError
+
+    x = to_dev(gpu, rng.integers(0, 1 << 64, size=(9, 2), dtype=np.uint64))
+    with pytest.raises(BnttError):
+        fx.hilo_to_u128(x[:8], out=x[1:])  # shifted overlap
 
 
 def test_golden_through_device_conversion(gpu, fixtures):

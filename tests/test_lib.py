@@ -41,6 +41,7 @@ def test_loads_and_versions():
     assert lib.bntt_version() == 100
     assert lib.bntt_ntt_tmp_bytes(12, 5) == 0
     assert lib.bntt_ntt_tmp_bytes(16, 3) == 3 * (1 << 16) * 16
+    assert lib.bntt_ntt_tmp_bytes(24, 1 << 40) == 2**64 - 1  # does not fit a size_t
 
 
 def test_no_cpu_fallback_without_gpu():
