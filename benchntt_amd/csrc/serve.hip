@@ -113,28 +113,36 @@ class Arena {
 constexpr size_t kUnit = 256;
 
 int passes_for(int L) { return L <= 11 ? 1 : (L + 7) / 8; }
-// Points per thread per register sub-pass, 2^elog: `want` (3 normally, 2 for
-// small batches: twice the items, each half as long), within what a size
-// allows (a one-pass transform must fit the 256 << elog values of an item).
-int elog_for(int L, int want)
+// Passes of a transform in a batch whose items hold 256 << want values: in
+// small batches (want < 3) sizes 2^10 and 2^11 take two passes.  In one pass
+// such a transform is a single item on one CU (about 12 and 20 us per level);
+// in two, each pass spreads it over several, in the levels that the batch's
+// larger sizes need anyway.
+int passes_in_batch(int L, int want) { return (want < 3 && L >= 10 && L <= 11) ? 2 : passes_for(L); }
+// Points per thread per register sub-pass, 2^elog: `want` (3 normally, 2 or 1
+// for small batches: more items, each shorter), within what the size allows
+// (a one-pass transform must fit the 256 << elog values of an item).
+int elog_for(int L, int want, int np)
 {
     int e = want < 3 ? want : 3;
-    if (passes_for(L) == 1 && e < L - 8) e = L - 8;
+    if (np == 1 && e < L - 8) e = L - 8;
     return e < L ? e : L;
 }
 
+// Arena units of a request in flight: FA and FB, plus a pass buffer each for
+// a and b for sizes that may take two passes.
 size_t request_units(int L)
 {
-    const size_t bufs = passes_for(L) == 1 ? 2 : 4;
+    const size_t bufs = L < 10 ? 2 : 4;
     return ((bufs << L) * sizeof(U128) + kUnit - 1) / kUnit;
 }
 
 // Upper bound of work items one request contributes (three transforms).
 size_t items_per_request(int L, int want)
 {
-    const int np = passes_for(L);
+    const int np = passes_in_batch(L, want);
     if (np == 1) return 3;
-    const int elog = elog_for(L, want);
+    const int elog = elog_for(L, want, np);
     int ks[3];
     bntt::pass_split(L, np, elog, 8 + elog, ks);
     size_t per_transform = 0;
@@ -287,11 +295,12 @@ struct ReqView {
     size_t arena_off;  // in units
 };
 
-// Transforms of one size sharing one table: packed several per item when single-pass.
+// Transforms of one size sharing one table, in np passes: packed several per
+// item when single-pass.
 void add_transforms(std::vector<std::vector<WorkItem>> &lv, int lv0, std::vector<DevDesc> &descs,
-                    const std::vector<DevDesc> &group, int L, uint8_t flags, int want)
+                    const std::vector<DevDesc> &group, int L, uint8_t flags, int want, int np)
 {
-    const int np = passes_for(L), elog = elog_for(L, want);
+    const int elog = elog_for(L, want, np);
     if (np == 1) {
         const int Glog = 8 + elog - L;
         for (size_t i = 0; i < group.size(); i += ((size_t)1 << Glog)) {
@@ -362,13 +371,14 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
     for (const ReqView &r : reqs) items8 += items_per_request(r.L, 3);
     const int want = want_for(s, items8);
     for (int L = s->min_l; L <= s->max_l; L++) {
+        const int np = passes_in_batch(L, want);
         std::vector<DevDesc> fwd, inv;
         for (const ReqView &r : reqs) {
             if (r.L != L) continue;
             const size_t n = (size_t)1 << L;
             U128 *base = arena + r.arena_off * (kUnit / sizeof(U128));
             U128 *FA = base, *FB = base + n;
-            U128 *TA = passes_for(L) > 1 ? base + 2 * n : nullptr, *TB = TA ? base + 3 * n : nullptr;
+            U128 *TA = np > 1 ? base + 2 * n : nullptr, *TB = TA ? base + 3 * n : nullptr;
             DevDesc da{}, db{}, di{};
             da.x = r.a, da.y = FA, da.tmp = TA, da.twr = s->tabs[L].fwd;
             db.x = r.b, db.y = FB, db.tmp = TB, db.twr = s->tabs[L].fwd;
@@ -379,8 +389,8 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
             inv.push_back(di);
         }
         if (!fwd.empty()) {
-            add_transforms(lv, 0, descs, fwd, L, 0, want);
-            add_transforms(lv, 2, descs, inv, L, bntt::IT_GATHER_MUL | bntt::IT_SCATTER_SCALE, want);
+            add_transforms(lv, 0, descs, fwd, L, 0, want, np);
+            add_transforms(lv, 2, descs, inv, L, bntt::IT_GATHER_MUL | bntt::IT_SCATTER_SCALE, want, np);
         }
     }
     return launch_levels(s, sl, lv, descs);
@@ -628,10 +638,10 @@ int bntt_pm_transform(bntt_pm_server *s, int direction, void *y, const void *x, 
         std::vector<DevDesc> descs;
         // radix-8 items of the chunk: whole transforms packed 2^Glog per item
         // for one-pass sizes, else blocks of one transform
-        const int glog8 = 8 + elog_for(L, 3) - L;
+        const int glog8 = 8 + elog_for(L, 3, 1) - L;
         const size_t items8 = passes_for(L) == 1 ? (m + ((size_t)1 << glog8) - 1) >> glog8
                                                  : m * (items_per_request(L, 3) / 3);
-        add_transforms(lv, 0, descs, group, L, flags, want_for(s, items8));
+        add_transforms(lv, 0, descs, group, L, flags, want_for(s, items8), passes_for(L));
         rc = launch_levels(s, sl, lv, descs);
         if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = bntt_set_error(BNTT_EHIP, "transform failed");
     }

@@ -199,7 +199,7 @@ typedef struct bntt_pm_request {
 
 typedef struct bntt_pm_config {
     uint64_t hbm_budget_bytes; /* arena for intermediates: per request in flight, 32*N bytes
-                                  for N <= 2^11 points, 64*N bytes above */
+                                  for N < 2^10 points, 64*N bytes from 2^10 on */
     int32_t max_inflight;      /* batches on the device at once (1..8 streams) */
     int32_t max_batch;         /* requests per batch */
     int32_t min_log2n;         /* sizes served: tables are built at creation */

@@ -18,8 +18,9 @@ UNIT = 256
 
 def req_bytes(L):
     """Arena bytes one request of 2^L points holds while in flight: two
-    buffers of N residues up to 2^11 points (one-pass transforms), four above."""
-    bufs = 2 if L <= 11 else 4
+    buffers of N residues below 2^10 points (always one-pass transforms), four
+    from 2^10 on (room for two passes)."""
+    bufs = 2 if L <= 9 else 4
     return -(-(bufs << L) * 16 // UNIT) * UNIT
 
 
