@@ -77,14 +77,20 @@ class PmConfig(ctypes.Structure):
     """bntt_pm_config."""
 
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
-                ("max_batch", ctypes.c_int32), ("min_log2n", ctypes.c_int32), ("max_log2n", ctypes.c_int32)]
+                ("max_batch", ctypes.c_int32), ("min_log2n", ctypes.c_int32), ("max_log2n", ctypes.c_int32),
+                ("admit_timeout_ns", ctypes.c_uint64)]
 
 
 class PmStats(ctypes.Structure):
     """bntt_pm_stats."""
 
     _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64),
-                ("max_inflight_requests", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64)]
+                ("max_inflight_requests", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
+                ("dropped", ctypes.c_uint64)]
+
+
+# done_ns of a request that bntt_pm_run_trace dropped
+BNTT_PM_DROPPED = 2**64 - 1
 
 
 BNTT_PLAN_GRAPH = 1

@@ -172,6 +172,7 @@ struct bntt_pm_server {
     Mod md{};
     HostMont hm;
     int min_l = 0, max_l = 0, max_batch = 0, nslots = 0;
+    uint64_t admit_timeout_ns = 0;  // 0: requests wait for admission as long as it takes
     SizeTables tabs[bntt::kMaxLog2 + 1];
     char *arena_d = nullptr;
     Arena arena;
@@ -510,6 +511,7 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
     s->max_l = cfg->max_log2n;
     s->max_batch = cfg->max_batch;
     s->nslots = cfg->max_inflight;
+    s->admit_timeout_ns = cfg->admit_timeout_ns;
     for (int L = s->min_l; L <= s->max_l; L++)
         if ((rc = build_tables(s, L))) {
             destroy_server(s);
@@ -701,6 +703,16 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
             sl.busy = false;
             progress = true;
         }
+        // drop requests that waited too long for admission (the oldest are
+        // at the front)
+        if (s->admit_timeout_ns)
+            while (!pending.empty() && t - arrival_ns[pending.front()] > s->admit_timeout_ns) {
+                done_ns[pending.front()] = BNTT_PM_DROPPED;
+                pending.pop_front();
+                st.dropped++;
+                completed++;
+                progress = true;
+            }
         // admit arrived requests into a free slot while the budget allows
         for (int k = 0; k < s->nslots && !pending.empty(); k++) {
             Slot &sl = s->slots[k];

@@ -3,7 +3,7 @@ bntt_pm_* in include/bntt.h).
 
     srv = PolymulServer(q=config.Q_116, hbm_budget=1 << 30)
     srv.multiply([(a, b, c), ...])              # one batch, returns when done
-    done_ns, stats = srv.run_trace(reqs, arrival_ns)
+    done_ns, stats = srv.run_trace(reqs, arrival_ns)  # DROPPED: not admitted in time
     srv.forward(y, x, n); srv.inverse(x, y, n)  # the pipeline's two transforms
 
 a, b, c are device tensors of N canonical residues (torch int64 (N, 2)); the
@@ -18,6 +18,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from ._lib import BNTT_PM_DROPPED as DROPPED  # noqa: F401  (done_ns of a dropped request)
 from ._lib import PmConfig, PmRequest, PmStats, U128
 from .ops import _dev_ptr, _nelem
 
@@ -42,8 +43,10 @@ def _requests(reqs):
 
 class PolymulServer:
     def __init__(self, q: int, hbm_budget: int = 1 << 30, max_inflight: int = 4, max_batch: int = 64,
-                 min_log2n: int = 1, max_log2n: int = 16):
-        cfg = PmConfig(hbm_budget, max_inflight, max_batch, min_log2n, max_log2n)
+                 min_log2n: int = 1, max_log2n: int = 16, admit_timeout_ns: int = 0):
+        """admit_timeout_ns: run_trace drops a request not admitted within
+        this time of its arrival (0: never)."""
+        cfg = PmConfig(hbm_budget, max_inflight, max_batch, min_log2n, max_log2n, admit_timeout_ns)
         h = ctypes.c_void_p()
         qq = U128.of(q)
         _lib.call("bntt_pm_server_create", ctypes.byref(h), ctypes.byref(qq), ctypes.byref(cfg))
@@ -87,7 +90,8 @@ class PolymulServer:
                   arrival.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(reqs),
                   done.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st))
         return done, {"requests": st.requests, "batches": st.batches,
-                      "max_inflight_requests": st.max_inflight_requests, "admission_waits": st.admission_waits}
+                      "max_inflight_requests": st.max_inflight_requests, "admission_waits": st.admission_waits,
+                      "dropped": st.dropped}
 
     def close(self) -> None:
         if self._h:

@@ -204,13 +204,19 @@ typedef struct bntt_pm_config {
     int32_t max_batch;         /* requests per batch */
     int32_t min_log2n;         /* sizes served: tables are built at creation */
     int32_t max_log2n;         /* <= 16 */
+    uint64_t admit_timeout_ns; /* bntt_pm_run_trace drops a request not admitted within this
+                                  time of its arrival (0: never) */
 } bntt_pm_config;
+
+/* done_ns of a request that bntt_pm_run_trace dropped (its c is untouched). */
+#define BNTT_PM_DROPPED UINT64_MAX
 
 typedef struct bntt_pm_stats {
     uint64_t requests;
     uint64_t batches;
     uint64_t max_inflight_requests; /* most requests on the device at once */
     uint64_t admission_waits;       /* requests that arrived and waited for budget */
+    uint64_t dropped;               /* requests not admitted within admit_timeout_ns */
 } bntt_pm_stats;
 
 typedef struct bntt_pm_server bntt_pm_server;
@@ -237,8 +243,9 @@ int bntt_pm_transform(bntt_pm_server *server, int direction, void *y, const void
 /* Serve requests that arrive at arrival_ns[i] (sorted, ns after the call
  * starts): arrived requests are admitted in order while the HBM budget has
  * room, batched, and launched on a free stream while earlier batches still
- * run; done_ns[i] receives the time request i was seen complete.  Returns when
- * all are done. */
+ * run; done_ns[i] receives the time request i was seen complete, or
+ * BNTT_PM_DROPPED if it waited longer than the server's admit_timeout_ns for
+ * admission.  Returns when every request is done or dropped. */
 int bntt_pm_run_trace(bntt_pm_server *server, const bntt_pm_request *reqs, const uint64_t *arrival_ns, size_t n,
                       uint64_t *done_ns, bntt_pm_stats *stats);
 

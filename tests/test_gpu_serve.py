@@ -166,12 +166,45 @@ def test_tight_budget_admission(gpu):
     mixed = [gpu.empty_like(a) for a, _ in dev]
     done, st = srv.run_trace([(a, b, c) for (a, b), c in zip(dev, mixed)], arrival)
     assert st["admission_waits"] > 0
+    assert st["dropped"] == 0  # no admission timeout set
     assert 1 <= st["max_inflight_requests"] <= 2
     assert st["batches"] >= 12
     for i, ((a, b), c) in enumerate(zip(dev, mixed)):
         alone = gpu.empty_like(a)
         srv.multiply([(a, b, alone)])
         assert gpu.equal(alone, c), f"request {i}"
+    srv.close()
+
+
+def test_admission_timeout_drops(gpu):
+    """Forced from the host: 200 requests of 2^12 points arrive at once under
+    a budget of one request in flight and a 2 ms admission timeout.  The
+    first request is admitted at once and completes; requests still waiting
+    after 2 ms are dropped: their done time is DROPPED and their c keeps the
+    value it had; every completed c equals the same request multiplied alone."""
+    from benchntt_amd.polymul import DROPPED
+
+    q = C.Q_116
+    n_req = 200
+    srv = server(q, hbm_budget=int(1.5 * req_bytes(12)), max_inflight=2, max_batch=4, min_log2n=12,
+                 max_log2n=12, admit_timeout_ns=2_000_000)
+    logs, hosts, dev, arrival = _trace(gpu, n_req, q, 21, 12, 12, 0)
+    sentinel = to_dev(gpu, np.full((1 << 12, 2), 7, dtype=np.uint64))
+    outs = [sentinel.clone() for _ in range(n_req)]
+    done, st = srv.run_trace([(a, b, c) for (a, b), c in zip(dev, outs)], arrival)
+    dropped = done == DROPPED
+    assert st["requests"] == n_req
+    assert st["dropped"] == int(dropped.sum())
+    assert 1 <= st["dropped"] < n_req
+    assert not dropped[0]
+    assert st["max_inflight_requests"] == 1
+    for i, ((a, b), c) in enumerate(zip(dev, outs)):
+        if dropped[i]:
+            assert gpu.equal(c, sentinel), f"dropped request {i} wrote its result"
+        else:
+            alone = gpu.empty_like(a)
+            srv.multiply([(a, b, alone)])
+            assert gpu.equal(alone, c), f"request {i}"
     srv.close()
 
 

@@ -1,5 +1,6 @@
 """CPU checks of the C ABI library: it loads, exports exactly what
 include/bntt.h declares, and fails loudly (no CPU fallback) without a GPU."""
+import ctypes
 import os
 import re
 import subprocess
@@ -56,6 +57,33 @@ def test_no_cpu_fallback_without_gpu():
         bn.vector_mul(a, a, a, bn.config.Q_BLAS, bn.config.MU_BLAS)
     with pytest.raises(bn.BnttError):
         bn.ntt(a, a, bn.config.Q_NTT, bn.config.MU_NTT, a)
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes structure of the Python binding has the size and field
+    offsets of its C struct in include/bntt.h (compiled with gcc)."""
+    from benchntt_amd import _lib
+
+    structs = {"bntt_u128": _lib.U128, "bntt_ntt_task": _lib.NttTask, "bntt_mul_task": _lib.MulTask,
+               "bntt_tune_entry": _lib.TuneEntry, "bntt_batch_info": _lib.BatchInfo,
+               "bntt_pm_request": _lib.PmRequest, "bntt_pm_config": _lib.PmConfig, "bntt_pm_stats": _lib.PmStats}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "bntt.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src), "-o",
+                    str(exe)], check=True)
+    got = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    want = []
+    for cname, py in structs.items():
+        want.append(f"{cname} size {ctypes.sizeof(py)}")
+        want += [f"{cname} {fname} {getattr(py, fname).offset}" for fname, _ in py._fields_]
+    assert got[:-1] == want
 
 
 def test_compat_header_compiles(tmp_path):
