@@ -163,6 +163,11 @@ struct Slot {
     bool busy = false;
     std::vector<size_t> reqs;                          // request indices of the batch
     std::vector<std::pair<size_t, size_t>> allocs;     // arena blocks
+    // work lists being built for the slot's next batch: kept across batches
+    // so that building one reuses their memory (host time per batch is what
+    // bounds the trace runner at middle loads)
+    std::vector<std::vector<WorkItem>> lv;
+    std::vector<DevDesc> descs, fwd, inv;
 };
 
 }  // namespace
@@ -365,15 +370,20 @@ int launch_batch(bntt_pm_server *s, Slot &sl, const std::vector<ReqView> &reqs)
 {
     // levels: 0, 1 forward passes; 2, 3 inverse passes (single-pass sizes use 0 and 2)
     constexpr int kLevels = 4;
-    std::vector<std::vector<WorkItem>> lv(kLevels);
-    std::vector<DevDesc> descs;
+    std::vector<std::vector<WorkItem>> &lv = sl.lv;
+    std::vector<DevDesc> &descs = sl.descs;
+    lv.resize(kLevels);
+    for (auto &v : lv) v.clear();
+    descs.clear();
     U128 *arena = reinterpret_cast<U128 *>(s->arena_d);
     size_t items8 = 0;
     for (const ReqView &r : reqs) items8 += items_per_request(r.L, 3);
     const int want = want_for(s, items8);
     for (int L = s->min_l; L <= s->max_l; L++) {
         const int np = passes_in_batch(L, want);
-        std::vector<DevDesc> fwd, inv;
+        std::vector<DevDesc> &fwd = sl.fwd, &inv = sl.inv;
+        fwd.clear();
+        inv.clear();
         for (const ReqView &r : reqs) {
             if (r.L != L) continue;
             const size_t n = (size_t)1 << L;
@@ -414,7 +424,8 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
     const size_t desc_off = nitems * sizeof(WorkItem);
     WorkItem *it_h = reinterpret_cast<WorkItem *>(sl.stage_h);
     DevDesc *de_h = reinterpret_cast<DevDesc *>(sl.stage_h + desc_off);
-    std::vector<size_t> off(kLevels);
+    size_t off[8];
+    if (kLevels > 8) return bntt_set_error(BNTT_EINVAL, "internal: too many levels");
     size_t pos = 0;
     for (int l = 0; l < kLevels; l++) {
         off[l] = pos;
@@ -427,14 +438,14 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
     if (e != hipSuccess) return bntt_set_hip_error(e, "hipMemcpyAsync of a batch");
     for (int l = 0; l < kLevels; l++) {
         if (lv[l].empty()) continue;
-        size_t data_slots = 0;
-        std::vector<size_t> needs;
+        // the largest data tile and staged-table need of the level's items
+        size_t data_slots = 0, need = 0;
         for (const WorkItem &w : lv[l]) {
             data_slots = std::max(data_slots, bntt::lds_data_slots(w.k, w.Glog));
-            needs.push_back(bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, w.Glog)));
+            need = std::max(need, bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, w.Glog)));
         }
         bntt::TwArea area;
-        const size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), s->tw_budget, &area);
+        const size_t lds = bntt::plan_tw_area(data_slots, &need, 1, s->tw_budget, &area);
         const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)(lds > bntt::kQueueLds3 ? s->grid_cap2 : s->grid_cap));
         launch_serving(s, dim3(grid), lds, sl.stream, reinterpret_cast<const WorkItem *>(sl.batch_d) + off[l],
                        (uint32_t)lv[l].size(), reinterpret_cast<const DevDesc *>(sl.batch_d + desc_off),
