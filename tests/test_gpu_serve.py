@@ -125,6 +125,23 @@ def test_large_sizes_oracle(gpu, orc, name, sizes):
     srv.close()
 
 
+@pytest.mark.parametrize("name", ["q116", "q122", "q64"])
+def test_small_sizes_one_and_two_passes(gpu, orc, name):
+    """Sizes 2^10 and 2^11 run in two passes in a small batch and in one pass
+    in a batch large enough for 2048-value items (here beside fifteen 2^16
+    requests); both equal the oracle's exact NTT pipeline, for the sparse
+    (q_116) and dense Montgomery reductions."""
+    q, _ = POLY_MODULI[name]
+    srv = server(q, min_log2n=10, max_log2n=16, max_batch=16)
+    big = [(rand(1 << 16, q, 900 + i), rand(1 << 16, q, 950 + i)) for i in range(15)]
+    for L in (10, 11):
+        a, b = rand(1 << L, q, 500 + L), rand(1 << L, q, 600 + L)
+        want = orc.polymul_ntt(a, b, q)
+        assert np.array_equal(run_batch(gpu, srv, [(a, b)])[0], want), f"N=2^{L} alone"
+        assert np.array_equal(run_batch(gpu, srv, [(a, b)] + big)[0], want), f"N=2^{L} in a large batch"
+    srv.close()
+
+
 def _trace(torch, n, q, seed, lo, hi, gap_ns):
     rng = np.random.default_rng(seed)
     logs = rng.integers(lo, hi + 1, size=n)
