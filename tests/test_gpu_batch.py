@@ -315,3 +315,51 @@ def test_contiguous_packed_items(gpu, orc):
             for i in lay[:: max(1, cnt // 7)] + [lay[-1]]:
                 assert np.array_equal(y[i * n:(i + 1) * n], orc.ntt(x[i * n:(i + 1) * n], tw, QN, MUN)), (L, i)
             plan.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_plans(gpu, orc, seed):
+    """Seeded random plans: 1..300 transforms of sizes 2^1..2^14 whose tables
+    are shared (a table of 2^14 also serves every smaller size: a transform of
+    2^L reads only entries below 2^L, and the plan converts one table once
+    for all its users), some transforms in place, products on
+    random pairs of equal size (a transform may feed several), executed from
+    a graph or not, each twiddle placement; every output against the oracle."""
+    from benchntt_amd.batch import BatchPlan
+
+    rng = np.random.default_rng(1000 + seed)
+    n_t = int(rng.integers(1, 301))
+    logs = [int(v) for v in rng.integers(1, 15, size=n_t)]
+    big = uniform(1 << 14, QN, 7000 + seed)
+    own = {L: uniform(1 << L, QN, 7100 + 20 * seed + L) for L in set(logs)}
+    dbig = to_dev(gpu, big)
+    down = {L: to_dev(gpu, t) for L, t in own.items()}
+    shared = [bool(rng.integers(0, 2)) for _ in logs]
+    inplace = [L <= 11 and bool(rng.integers(0, 3) == 0) for L in logs]
+    xs = [uniform(1 << L, QN, 8000 + 500 * seed + i) for i, L in enumerate(logs)]
+    dxs = [to_dev(gpu, x) for x in xs]
+    dys = [dx if ip else gpu.empty_like(dx) for dx, ip in zip(dxs, inplace)]
+    # a shared table is passed as its prefix view: the same device pointer
+    tasks = [(dx, dy, dbig[:1 << L] if sh else down[L]) for dx, dy, sh, L in zip(dxs, dys, shared, logs)]
+    by_size = {}
+    for i, L in enumerate(logs):
+        by_size.setdefault(L, []).append(i)
+    pairs = []
+    for L, idx in by_size.items():
+        for _ in range(len(idx) // 2 + 1):
+            i, j = (int(v) for v in rng.choice(idx, size=2))
+            pairs.append((i, j))
+    dcs = [gpu.empty_like(dys[i]) for i, _ in pairs]
+    muls = [(dys[i], dys[j], dc) for (i, j), dc in zip(pairs, dcs)]
+    graph = seed % 2 == 0
+    tw_mode = ("global", "lds", "wide")[seed % 3]
+    print(f"\nrandom plan seed {seed}: {n_t} transforms, {len(pairs)} products, graph {graph}, {tw_mode}", flush=True)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=graph, tw_mode=tw_mode)
+    plan.execute()
+    gpu.cuda.synchronize()
+    want = [orc.ntt(x, (big if sh else own[L])[:1 << L], QN, MUN) for x, sh, L in zip(xs, shared, logs)]
+    for i, dy in enumerate(dys):
+        assert np.array_equal(from_dev(dy), want[i]), f"transform {i} (N=2^{logs[i]})"
+    for k, ((i, j), dc) in enumerate(zip(pairs, dcs)):
+        assert np.array_equal(from_dev(dc), orc.vvmul(want[i], want[j], QN, MUN)), f"product {k}"
+    plan.close()
