@@ -152,11 +152,13 @@ def _trace(torch, n, q, seed, lo, hi, gap_ns):
     return logs, hosts, dev, arrival
 
 
-def test_alone_equals_mixed(gpu, orc):
+@pytest.mark.parametrize("name", ["q116", "q122"])
+def test_alone_equals_mixed(gpu, orc, name):
     """Each request of a Poisson trace with mixed sizes 2^4 .. 2^14, served
     with continuous batching over several streams, gives the same bits as the
-    same request multiplied alone; a sample is checked against the oracle."""
-    q = C.Q_116
+    same request multiplied alone; a sample is checked against the oracle.
+    q_116 takes the sparse Montgomery reduction, q122 the dense one."""
+    q, _ = POLY_MODULI[name]
     srv = server(q, hbm_budget=1 << 30, max_inflight=4, max_batch=16, min_log2n=4, max_log2n=14)
     logs, hosts, dev, arrival = _trace(gpu, 160, q, 7, 4, 14, 20_000)
     mixed = [gpu.empty_like(a) for a, _ in dev]
