@@ -327,11 +327,15 @@ int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t c
         return fail(BNTT_EINVAL, "device arrays must be 16-byte aligned");
     if (count > (SIZE_MAX / sizeof(bntt_u128)) >> log2n) return fail(BNTT_EINVAL, "count = %zu is too large", count);
     const size_t n = (size_t)1 << log2n;
+    const size_t bytes = count * n * sizeof(bntt_u128);
     if (log2n > bntt::kLdsMaxLog2) {
         if (!tmp) return fail(BNTT_EINVAL, "log2n = %d needs a tmp buffer of bntt_ntt_tmp_bytes()", log2n);
         const char *yb = static_cast<const char *>(y), *xb = static_cast<const char *>(x);
-        const size_t bytes = count * n * sizeof(bntt_u128);
         if (yb < xb + bytes && xb < yb + bytes) return fail(BNTT_EINVAL, "y must not overlap x for log2n > 12");
+    } else if (partial_overlap(y, x, bytes)) {
+        // one workgroup's stores would land in another transform's input
+        // before that transform is loaded
+        return fail(BNTT_EINVAL, "y must equal x or not overlap it");
     }
     const Mod md = make_mod(m, mu);
     hipError_t e = bntt::launch_ntt_uniform(static_cast<U128 *>(y), static_cast<const U128 *>(x),

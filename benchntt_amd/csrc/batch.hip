@@ -110,6 +110,13 @@ struct bntt_batch_plan {
     bool use_graph = false;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    // Executions share the work-queue counters, punt lists, intermediates and
+    // digit tables, so they must not overlap on the device: each execution
+    // outside a stream capture records `done` on its stream, and the next one
+    // on another stream waits for it first.
+    hipEvent_t done = nullptr;
+    hipStream_t last_stream = nullptr;
+    bool executed = false;
     std::mutex mu;
 };
 
@@ -154,6 +161,7 @@ int upload(T **dst, const std::vector<T> &v)
 void free_plan(bntt_batch_plan *p)
 {
     if (!p) return;
+    if (p->done) (void)hipEventDestroy(p->done);
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     (void)hipFree(p->items_d);
@@ -242,6 +250,10 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     }
     p->md = bntt_make_mod(m, mu);
     p->use_graph = (flags & BNTT_PLAN_GRAPH) != 0;
+    if (hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess) {
+        free_plan(p);
+        return bntt_set_error(BNTT_EHIP, "hipEventCreateWithFlags");
+    }
 
     // descriptors: one per transform; multi-pass transforms get tmp space
     std::vector<DevDesc> descs(ntasks);
@@ -608,31 +620,45 @@ int bntt_batch_plan_execute(bntt_batch_plan *p, void *stream)
     hipStream_t s = static_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(p->mu);
     // inside the caller's own stream capture the launches become part of the
-    // caller's graph
+    // caller's graph (whose replays the caller orders)
     hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
     if (s != nullptr && hipStreamIsCapturing(s, &cs) != hipSuccess)
         return bntt_set_error(BNTT_EHIP, "hipStreamIsCapturing");
-    if (!p->use_graph || s == nullptr || cs != hipStreamCaptureStatusNone) return launch_levels(p, s);
-    if (!p->exec) {
-        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
-        if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamBeginCapture");
-        const int rc = launch_levels(p, s);
-        hipGraph_t g = nullptr;
-        e = hipStreamEndCapture(s, &g);
-        if (rc) {
-            if (g) (void)hipGraphDestroy(g);
-            return rc;
-        }
-        if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamEndCapture");
-        e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(g);
-            return bntt_set_hip_error(e, "hipGraphInstantiate");
-        }
-        p->graph = g;
+    if (cs != hipStreamCaptureStatusNone) return launch_levels(p, s);
+    hipError_t e;
+    if (p->executed && p->last_stream != s) {
+        e = hipStreamWaitEvent(s, p->done, 0);
+        if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamWaitEvent");
     }
-    const hipError_t e = hipGraphLaunch(p->exec, s);
-    if (e != hipSuccess) return bntt_set_hip_error(e, "hipGraphLaunch");
+    if (!p->use_graph || s == nullptr) {
+        const int rc = launch_levels(p, s);
+        if (rc) return rc;
+    } else {
+        if (!p->exec) {
+            e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamBeginCapture");
+            const int rc = launch_levels(p, s);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(s, &g);
+            if (rc) {
+                if (g) (void)hipGraphDestroy(g);
+                return rc;
+            }
+            if (e != hipSuccess) return bntt_set_hip_error(e, "hipStreamEndCapture");
+            e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                (void)hipGraphDestroy(g);
+                return bntt_set_hip_error(e, "hipGraphInstantiate");
+            }
+            p->graph = g;
+        }
+        e = hipGraphLaunch(p->exec, s);
+        if (e != hipSuccess) return bntt_set_hip_error(e, "hipGraphLaunch");
+    }
+    e = hipEventRecord(p->done, s);
+    if (e != hipSuccess) return bntt_set_hip_error(e, "hipEventRecord");
+    p->last_stream = s;
+    p->executed = true;
     return BNTT_OK;
 }
 

@@ -597,6 +597,9 @@ int bntt_pm_multiply(bntt_pm_server *s, const bntt_pm_request *reqs, size_t n)
     if (!rc && !views.empty()) {
         rc = launch_batch(s, sl, views);
         if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = bntt_set_error(BNTT_EHIP, "batch failed");
+        // a launch that failed part-way may have enqueued kernels that write
+        // the blocks released below
+        if (rc) (void)hipStreamSynchronize(sl.stream);
     }
     for (auto &a : allocs) s->arena.release(a.first, a.second);
     return rc;
@@ -658,6 +661,7 @@ int bntt_pm_transform(bntt_pm_server *s, int direction, void *y, const void *x, 
         rc = launch_levels(s, sl, lv, descs);
         if (!rc && hipEventSynchronize(sl.done) != hipSuccess) rc = bntt_set_error(BNTT_EHIP, "transform failed");
     }
+    if (rc) (void)hipStreamSynchronize(sl.stream);  // nothing enqueued may still write the block
     if (per) s->arena.release(off, chunk * per);
     return rc;
 }
@@ -685,7 +689,10 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
     auto abandon = [&](int rc) {
         for (int k = 0; k < s->nslots; k++) {
             Slot &sl = s->slots[k];
-            if (sl.busy) (void)hipStreamSynchronize(sl.stream);
+            // a slot whose launch failed part-way is not marked busy but may
+            // have kernels enqueued that write its arena blocks: wait for its
+            // stream before the blocks are handed out again
+            if (sl.busy || !sl.allocs.empty()) (void)hipStreamSynchronize(sl.stream);
             for (auto &a : sl.allocs) s->arena.release(a.first, a.second);
             sl.allocs.clear();
             sl.reqs.clear();

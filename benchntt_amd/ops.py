@@ -54,7 +54,8 @@ def _host_ptr(a: np.ndarray) -> int:
 
 def _stream(stream=None):
     if stream is not None:
-        return ctypes.c_void_p(int(stream))
+        # a torch stream object or a raw hipStream_t handle
+        return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
     import torch
 
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
@@ -132,16 +133,25 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
     if _mode(Y, X, twiddles) == "dev":
         need = tmp_elements(log2n, count)
         tmp = 0
+        own = None
         if need:
             if as0 is None:
                 import torch
 
-                as0 = torch.empty((need, 2), dtype=torch.int64, device=X.device)
+                as0 = own = torch.empty((need, 2), dtype=torch.int64, device=X.device)
             if _nelem(as0) < need:
                 raise ValueError(f"as0 must hold {need} residues for this N")
             tmp = _dev_ptr(as0)
+        s = _stream(stream)
         _lib.call("bntt_ntt_fwd_dev", _dev_ptr(Y), _dev_ptr(X), _dev_ptr(twiddles), log2n, count,
-                  ctypes.byref(mm), ctypes.byref(mmu), tmp, _stream(stream))
+                  ctypes.byref(mm), ctypes.byref(mmu), tmp, s)
+        if own is not None and stream is not None:
+            # the scratch was allocated on torch's current stream but is
+            # written on `stream`: keep the allocator from reusing it before
+            # the launch's kernels have run
+            import torch
+
+            own.record_stream(torch.cuda.ExternalStream(s.value or 0, device=X.device))
     else:
         for i in range(count):
             off = i * n * 16

@@ -160,10 +160,14 @@ typedef struct bntt_batch_plan bntt_batch_plan;
 int bntt_batch_plan_create(bntt_batch_plan **plan, const bntt_ntt_task *tasks, size_t ntasks,
                            const bntt_mul_task *muls, size_t nmuls, const bntt_u128 *m, const bntt_u128 *mu,
                            const bntt_tune_entry *tune, size_t ntune, int flags);
-/* Enqueue one execution on `stream` (no synchronization).  On a stream that is
- * being captured, the launches join the caller's graph, which may then be
- * replayed any number of times (no memset nodes; the work-queue counters
- * rearm themselves). */
+/* Enqueue one execution on `stream` (no host synchronization).  Executions of
+ * one plan share its work-queue counters and intermediates, so one runs at a
+ * time on the device: an execution enqueued on another stream than the
+ * previous one first waits (on the device) for the previous one to finish.
+ * On a stream that is being captured, the launches join the caller's graph,
+ * which may then be replayed any number of times (no memset nodes; the
+ * work-queue counters rearm themselves); the caller must not let replays of
+ * that graph overlap each other or other executions of the plan. */
 int bntt_batch_plan_execute(bntt_batch_plan *plan, void *stream);
 int bntt_batch_plan_info(const bntt_batch_plan *plan, bntt_batch_info *info);
 /* Synchronizes the device, then frees the plan. */

@@ -363,3 +363,31 @@ def test_random_plans(gpu, orc, seed):
     for k, ((i, j), dc) in enumerate(zip(pairs, dcs)):
         assert np.array_equal(from_dev(dc), orc.vvmul(want[i], want[j], QN, MUN)), f"product {k}"
     plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_executions_on_two_streams_in_order(gpu, orc, graph):
+    """Executions of one plan enqueued on two streams without host
+    synchronisation run one after the other (the second stream waits for the
+    first execution): the shared work-queue counters stay consistent, so a
+    later execution after the outputs are cleared is still complete."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(60, QN, seed=77, lo=4, hi=14)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    s1, s2 = gpu.cuda.Stream(), gpu.cuda.Stream()
+    with gpu.cuda.stream(s1):
+        plan = BatchPlan(tasks, muls, QN, MUN, graph=graph)
+    gpu.cuda.synchronize()
+    for _ in range(3):
+        plan.execute(s1)
+        plan.execute(s2)
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc))
+    dy.zero_()
+    dc.zero_()
+    gpu.cuda.synchronize()
+    plan.execute(s2)
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc))
+    plan.close()

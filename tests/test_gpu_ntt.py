@@ -70,6 +70,21 @@ def test_in_place(gpu, orc, L):
     assert np.array_equal(ntt_dev(gpu, x, tw, inplace=True), orc.ntt(x, tw, QN, MUN))
 
 
+@pytest.mark.parametrize("L", [6, 10, 12])
+def test_shifted_overlap_rejected(gpu, L):
+    """y = x + N (a shifted overlap) is an argument error for one-pass sizes
+    too: one workgroup's stores would land in another transform's input before
+    that transform is loaded.  Nothing is launched."""
+    import benchntt_amd as bn
+
+    n = 1 << L
+    buf = gpu.zeros((6 * n, 2), dtype=gpu.int64, device="cuda")
+    tw = gpu.zeros((n, 2), dtype=gpu.int64, device="cuda")
+    with pytest.raises(bn.BnttError) as ei:
+        bn.ntt(buf[n:5 * n], buf[:4 * n], QN, MUN, tw)
+    assert ei.value.code == 1 and "overlap" in str(ei.value)
+
+
 def test_edge_inputs(gpu, orc):
     """Inputs and twiddles at q, above q and above 2^124 follow the reference
     formulas exactly (the fast reduction hands such operands to the literal
