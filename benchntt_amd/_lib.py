@@ -95,6 +95,9 @@ BNTT_PM_DROPPED = 2**64 - 1
 
 BNTT_PLAN_GRAPH = 1
 
+# residues of the fixed scratch of bntt_ntt_fwd_dev_noalloc (BNTT_NOALLOC_MAX_RESIDUES)
+BNTT_NOALLOC_MAX_RESIDUES = 1 << 20
+
 _P = ctypes.c_void_p
 _PU = ctypes.POINTER(U128)
 _SZ = ctypes.c_size_t
@@ -108,6 +111,7 @@ _SIGS = {
     "bntt_vvmul_dev": [_P, _P, _P, _SZ, _PU, _PU, _P],
     "bntt_axpy_dev": [_P, _P, _P, _PU, _SZ, _PU, _PU, _P],
     "bntt_ntt_fwd_dev": [_P, _P, _P, ctypes.c_int, _SZ, _PU, _PU, _P, _P],
+    "bntt_ntt_fwd_dev_noalloc": [_P, _P, _P, ctypes.c_int, _SZ, _PU, _PU, _P],
     "bntt_vvadd": [_P, _P, _P, _SZ, _PU],
     "bntt_vvsub": [_P, _P, _P, _SZ, _PU],
     "bntt_vvmul": [_P, _P, _P, _SZ, _PU, _PU],

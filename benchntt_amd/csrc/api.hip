@@ -83,8 +83,20 @@ Mod make_mod(const bntt_u128 *m, const bntt_u128 *mu)
     return md;
 }
 
+// Intermediates of bntt_ntt_fwd_dev_noalloc: a fixed array in device memory,
+// one copy per GPU, held until the process ends.  Host code never uses the
+// array's name as an address: the device address of each GPU's copy comes
+// from hipGetSymbolAddress at that device's init.
+constexpr size_t kNoallocResidues = BNTT_NOALLOC_MAX_RESIDUES;
+static __device__ __align__(256) uint32_t scratch[4 * kNoallocResidues];
+
+// Per device: whether it was checked to be gfx950, and its scratch address.
+struct DevState {
+    bool ok = false;
+    void *scratch = nullptr;
+};
 std::mutex g_init_mu;
-int g_init_dev = -1;
+std::map<int, DevState> g_devs;
 
 int ensure_init()
 {
@@ -92,13 +104,34 @@ int ensure_init()
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     std::lock_guard<std::mutex> lk(g_init_mu);
-    if (g_init_dev == dev) return BNTT_OK;
+    DevState &ds = g_devs[dev];
+    if (ds.ok) return BNTT_OK;
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return fail(BNTT_ENODEV, "device %d is %s; this library is built for gfx950 only", dev, prop.gcnArchName);
-    g_init_dev = dev;
+    // this device's copy of the scratch array (device `dev` is current)
+    void *sp = nullptr;
+    e = hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!sp) return fail(BNTT_EHIP, "hipGetSymbolAddress(scratch) gave a null address");
+    ds.scratch = sp;
+    ds.ok = true;
+    return BNTT_OK;
+}
+
+// The current device's scratch address (after ensure_init on that device).
+int noalloc_scratch(void **out)
+{
+    int dev = 0;
+    const hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    std::lock_guard<std::mutex> lk(g_init_mu);
+    auto it = g_devs.find(dev);
+    if (it == g_devs.end() || !it->second.ok || !it->second.scratch)
+        return fail(BNTT_EHIP, "device %d is not initialised", dev);
+    *out = it->second.scratch;
     return BNTT_OK;
 }
 
@@ -343,6 +376,28 @@ int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t c
                                             static_cast<U128 *>(tmp), static_cast<hipStream_t>(stream));
     if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
     return BNTT_OK;
+}
+
+int bntt_ntt_fwd_dev_noalloc(void *y, const void *x, const void *tw, int log2n, size_t count, const bntt_u128 *m,
+                             const bntt_u128 *mu, void *stream)
+{
+    int rc = ensure_init();
+    if (rc) return rc;
+    if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2)
+        return fail(BNTT_EINVAL, "log2n = %d outside [%d, %d]", log2n, bntt::kMinLog2, bntt::kMaxLog2);
+    void *tmp = nullptr;
+    if (log2n > bntt::kLdsMaxLog2 && count > 0) {
+        // multi-pass sizes keep their intermediates in the fixed scratch array;
+        // every element of it that a later pass reads is written by an earlier
+        // pass of the same call, so nothing needs clearing
+        if (count > (kNoallocResidues >> log2n))
+            return fail(BNTT_EINVAL,
+                        "count * 2^log2n = %zu * 2^%d exceeds the %zu residues of the no-allocation scratch "
+                        "(use bntt_ntt_fwd_dev with a tmp buffer)",
+                        count, log2n, kNoallocResidues);
+        if ((rc = noalloc_scratch(&tmp))) return rc;
+    }
+    return bntt_ntt_fwd_dev(y, x, tw, log2n, count, m, mu, tmp, stream);
 }
 
 int bntt_vvadd(bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, size_t n, const bntt_u128 *m)
