@@ -211,7 +211,7 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
         twl = torch.from_numpy(uniform(n, qn, 40 + L).view(np.int64)).to(dev)
         tasks = [(xb[i * n:(i + 1) * n], yb[i * n:(i + 1) * n], twl) for i in range(cnt)]
         with torch.cuda.stream(s2):
-            plan = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True)
+            plan = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True, fixed_tables=True)
             msp = time_kernel(torch, plan.execute, 20, 3, s2)
             plan.close()
             if L == 10:
@@ -284,8 +284,8 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     s2 = torch.cuda.Stream()
     out = {"uneven_total_coefficients": wl.total, "uneven_tuned": bool(tune)}
     with torch.cuda.stream(s2):
-        full = BatchPlan(tasks, muls, qn, mun, tune=tune, graph=True)
-        only = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True)
+        full = BatchPlan(tasks, muls, qn, mun, tune=tune, graph=True, fixed_tables=True)
+        only = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True, fixed_tables=True)
         ms_full = time_kernel(torch, full.execute, 5, 2, s2)
         ms_only = time_kernel(torch, only.execute, 5, 2, s2)
         tmp = torch.empty((1 << 16, 2), dtype=torch.int64, device=dev)

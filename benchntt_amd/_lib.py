@@ -94,6 +94,7 @@ BNTT_PM_DROPPED = 2**64 - 1
 
 
 BNTT_PLAN_GRAPH = 1
+BNTT_PLAN_TW_FIXED = 8
 
 # residues of the fixed scratch of bntt_ntt_fwd_dev_noalloc (BNTT_NOALLOC_MAX_RESIDUES)
 BNTT_NOALLOC_MAX_RESIDUES = 1 << 20

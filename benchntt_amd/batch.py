@@ -66,10 +66,14 @@ class BatchPlan:
     """bntt_batch_plan_*: tasks are (x, y, tw) device tensors, muls (a, b, c);
     tw_mode "global" (twiddles from a digit copy in device memory), "lds"
     (staged in LDS where they keep three workgroups per CU) or "wide" (staged
-    up to 80 KiB of LDS)."""
+    up to 80 KiB of LDS).  fixed_tables: the twiddle tables are not modified
+    after creation, so they are converted once (BNTT_PLAN_TW_FIXED) instead of
+    at every execution.  A product whose operands are exactly the outputs of
+    two transforms of its size and one table is computed inside their last
+    pass."""
 
     def __init__(self, tasks, muls=(), modulus: int = 0, mu: int = 0, tune=None, graph: bool = True,
-                 tw_mode: str = "global"):
+                 tw_mode: str = "global", fixed_tables: bool = False):
         if tw_mode not in TW_FLAGS:
             raise ValueError(f"tw_mode must be one of {sorted(TW_FLAGS)}")
         self._keep = (list(tasks), list(muls))  # the buffers must outlive the plan
@@ -85,7 +89,8 @@ class BatchPlan:
         mm, mmu = U128.of(modulus), U128.of(mu)
         _lib.call("bntt_batch_plan_create", ctypes.byref(handle), tarr, len(self._keep[0]), marr,
                   len(self._keep[1]), ctypes.byref(mm), ctypes.byref(mmu), tune_arr, ntune,
-                  (_lib.BNTT_PLAN_GRAPH if graph else 0) | TW_FLAGS[tw_mode])
+                  (_lib.BNTT_PLAN_GRAPH if graph else 0) | TW_FLAGS[tw_mode]
+                  | (_lib.BNTT_PLAN_TW_FIXED if fixed_tables else 0))
         self._h = handle
 
     def execute(self, stream=None) -> None:

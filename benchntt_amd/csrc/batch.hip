@@ -51,13 +51,13 @@ static constexpr double kStaggerShare = BNTT_STAGGER_SHARE;
 
 namespace bntt {
 
-// Clears the table flags and the punt counts at the start of an execution (a
-// kernel rather than memsets, so that graph replays clear them too).
-__global__ __launch_bounds__(256) void exec_clear_kernel(uint32_t *flags, uint32_t n, uint32_t *counts, uint32_t nc)
+// Clears the table flags before the twiddle tables are converted (a kernel
+// rather than a memset, so that graph replays clear them too).  The punt
+// counts need no clearing: each slow launch clears the count it consumed.
+__global__ __launch_bounds__(256) void exec_clear_kernel(uint32_t *flags, uint32_t n)
 {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) flags[i] = 0;
-    if (i < nc) counts[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void tw_convert_kernel(const TwChunk *__restrict__ chunks,
@@ -93,6 +93,8 @@ struct bntt_batch_plan {
     std::vector<size_t> level_lds;       // dynamic LDS bytes of each level's launch
     std::vector<bntt::TwArea> level_tw;  // staged-twiddle area of each level's launch
     std::vector<int> level_grid;         // workgroups of each level's launch
+    std::vector<int> level_kind;         // fast kernel of each level (QK_FAST or QK_FAST8)
+    bool fixed_tw = false;               // digit tables made once, at creation
     WorkItem *items_d = nullptr;
     DevDesc *descs_d = nullptr;
     DevMul *muls_d = nullptr;
@@ -143,6 +145,17 @@ bool valid_choice(int L, const TuneChoice &c)
     return bntt::pass_split(L, c.np, c.elog, tile_log, ks);
 }
 
+// Whether two transforms of size 2^L can share the tiles of their last pass
+// (it leaves at least two groups per tile).
+bool pairable(int L, const TuneChoice &c)
+{
+    const int tile_log = 8 + c.elog;
+    if (c.np == 1) return L < tile_log;
+    int ks[3];
+    if (!bntt::pass_split(L, c.np, c.elog, tile_log, ks)) return false;
+    return ks[c.np - 1] < tile_log;
+}
+
 }  // namespace
 
 namespace {
@@ -177,21 +190,27 @@ void free_plan(bntt_batch_plan *p)
     delete p;
 }
 
+// Digit copies and large-entry flags of the plan's twiddle tables.
+int convert_tables(bntt_batch_plan *p, hipStream_t s)
+{
+    if (!p->ntwjobs) return BNTT_OK;
+    hipLaunchKernelGGL(bntt::exec_clear_kernel, dim3((p->ntwjobs + 255) / 256), dim3(256), 0, s, p->twflag_d,
+                       p->ntwjobs);
+    hipLaunchKernelGGL(bntt::tw_convert_kernel, dim3(p->ntwchunks), dim3(256), 0, s, p->twchunks_d, p->twjobs_d);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return bntt_set_hip_error(e, "twiddle conversion launch");
+    return BNTT_OK;
+}
+
 int launch_levels(bntt_batch_plan *p, hipStream_t s)
 {
     // each level: the fast launch, then a slow launch over the items it left to
     // the literal formulas (usually none); a modulus not of the fast form runs
     // every level on the literal formulas
     const uint32_t nlev = (uint32_t)p->level_items.size();
-    const uint32_t nclear = std::max<uint32_t>(p->ntwjobs, nlev);
-    if (nclear) {
-        hipLaunchKernelGGL(bntt::exec_clear_kernel, dim3((nclear + 255) / 256), dim3(256), 0, s, p->twflag_d,
-                           p->ntwjobs, p->punt_d, nlev);
-        if (p->ntwjobs)
-            hipLaunchKernelGGL(bntt::tw_convert_kernel, dim3(p->ntwchunks), dim3(256), 0, s, p->twchunks_d,
-                               p->twjobs_d);
-        const hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return bntt_set_hip_error(e, "twiddle conversion launch");
+    if (!p->fixed_tw) {
+        const int rc = convert_tables(p, s);
+        if (rc) return rc;
     }
     for (size_t l = 0; l < nlev; l++) {
         if (p->level_items[l] == 0) continue;
@@ -199,9 +218,14 @@ int launch_levels(bntt_batch_plan *p, hipStream_t s)
         const uint32_t n = (uint32_t)p->level_items[l];
         if (p->md.pm) {
             const bntt::PuntList punt{p->punt_d + l, p->punt_d + nlev + p->level_offset[l]};
-            hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_FAST>, dim3(p->level_grid[l]), dim3(bntt::kQueueThreads),
-                               p->level_lds[l], s, items, n, p->descs_d, p->muls_d, p->ctl_d, p->level_tw[l], p->md,
-                               punt);
+            if (p->level_kind[l] == bntt::QK_FAST8)
+                hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_FAST8>, dim3(p->level_grid[l]),
+                                   dim3(bntt::kQueueThreads), p->level_lds[l], s, items, n, p->descs_d, p->muls_d,
+                                   p->ctl_d, p->level_tw[l], p->md, punt);
+            else
+                hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_FAST>, dim3(p->level_grid[l]),
+                                   dim3(bntt::kQueueThreads), p->level_lds[l], s, items, n, p->descs_d, p->muls_d,
+                                   p->ctl_d, p->level_tw[l], p->md, punt);
             hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_SLOW>, dim3(p->slow_grid), dim3(bntt::kQueueThreads),
                                p->level_lds[l], s, items, 0u, p->descs_d, p->muls_d, p->ctl_d + 1, bntt::TwArea{},
                                p->md, punt);
@@ -250,6 +274,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     }
     p->md = bntt_make_mod(m, mu);
     p->use_graph = (flags & BNTT_PLAN_GRAPH) != 0;
+    p->fixed_tw = (flags & BNTT_PLAN_TW_FIXED) != 0;
     if (hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess) {
         free_plan(p);
         return bntt_set_error(BNTT_EHIP, "hipEventCreateWithFlags");
@@ -349,6 +374,35 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             if (choice[tasks[i].log2n].np > 1) descs[i].tmp = p->tmp_d + tmp_off[i];
     }
 
+    // Products fused into their transforms' last pass: c = y_A * y_B where A
+    // and B are transforms of one size and one table whose outputs are exactly
+    // the product's operands, each transform in at most one pair, and whose
+    // passes all leave room for two groups per tile.
+    std::vector<int64_t> partner(ntasks, -1);  // the other transform of a pair
+    std::vector<int64_t> pair_mul(ntasks, -1); // the pair's product (on its A)
+    std::vector<char> fused(nmuls, 0);
+    size_t nunfused = nmuls;
+    {
+        std::unordered_map<uintptr_t, size_t> by_y;
+        for (size_t j = 0; j < ntasks; j++) by_y.emplace(reinterpret_cast<uintptr_t>(tasks[j].y), j);
+        for (size_t i = 0; i < nmuls; i++) {
+            const uint64_t n = muls[i].n;
+            if (n < 2 || (n & (n - 1)) != 0) continue;
+            const int L = 63 - __builtin_clzll(n);
+            auto fa = by_y.find(reinterpret_cast<uintptr_t>(muls[i].a));
+            auto fb = by_y.find(reinterpret_cast<uintptr_t>(muls[i].b));
+            if (fa == by_y.end() || fb == by_y.end() || fa->second == fb->second) continue;
+            const size_t ia = fa->second, ib = fb->second;
+            if (tasks[ia].log2n != L || tasks[ib].log2n != L || tasks[ia].tw != tasks[ib].tw) continue;
+            if (partner[ia] >= 0 || partner[ib] >= 0 || !pairable(L, choice[L])) continue;
+            partner[ia] = (int64_t)ib;
+            partner[ib] = (int64_t)ia;
+            pair_mul[ia] = (int64_t)i;
+            fused[i] = 1;
+            nunfused--;
+        }
+    }
+
     // transform items, by level; single-pass transforms of one size and one
     // twiddle table are packed several per item (descriptors made contiguous)
     std::vector<std::vector<WorkItem>> ntt_lv(4), mul_lv(5);
@@ -361,13 +415,33 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         return reinterpret_cast<uintptr_t>(tasks[a].tw) < reinterpret_cast<uintptr_t>(tasks[b].tw);
     });
     std::vector<int> out_level(ntasks, 0);  // level at which each transform's output is final
-    // With products to run, multi-pass transforms start in two groups: the
-    // first kStaggerShare of their work (largest first) at level 0, the rest
-    // one level later.  The products of the first group then run beside the
-    // last pass of the second (memory-bound items among compute-bound ones)
-    // instead of all waiting for a level of their own after every transform.
+    // With unfused products to run, multi-pass transforms start in two
+    // groups: the first kStaggerShare of their work (largest first) at level
+    // 0, the rest one level later.  The products of the first group then run
+    // beside the last pass of the second (memory-bound items among
+    // compute-bound ones) instead of all waiting for a level of their own after
+    // every transform.
     std::vector<int> start_level(ntasks, 0);
-    if (nmuls > 0) {
+    bool stagger = false;  // some unfused product reads a multi-pass transform's output
+    if (nunfused > 0) {
+        std::map<uintptr_t, uintptr_t> multi;  // y -> end, multi-pass transforms
+        for (size_t j = 0; j < ntasks; j++)
+            if (choice[tasks[j].log2n].np > 1) {
+                const uintptr_t y = reinterpret_cast<uintptr_t>(tasks[j].y);
+                multi[y] = y + (sizeof(U128) << tasks[j].log2n);
+            }
+        auto reads_multi = [&](const void *a, uint64_t n) {
+            const uintptr_t lo = reinterpret_cast<uintptr_t>(a), hi = lo + sizeof(U128) * n;
+            auto it = multi.upper_bound(lo);
+            if (it != multi.begin()) --it;
+            for (; it != multi.end() && it->first < hi; ++it)
+                if (it->second > lo) return true;
+            return false;
+        };
+        for (size_t i = 0; i < nmuls && !stagger; i++)
+            stagger = !fused[i] && (reads_multi(muls[i].a, muls[i].n) || reads_multi(muls[i].b, muls[i].n));
+    }
+    if (stagger) {
         auto work = [&](size_t j) { return (double)tasks[j].log2n * (double)((size_t)1 << tasks[j].log2n); };
         double total = 0, acc = 0;
         for (size_t j : order)
@@ -377,18 +451,33 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
                 if (acc >= kStaggerShare * total) start_level[j] = 1;
                 acc += work(j);
             }
+        // the two transforms of a pair start together
+        for (size_t j = 0; j < ntasks; j++)
+            if (pair_mul[j] >= 0) start_level[(size_t)partner[j]] = start_level[j];
     }
+    auto with_prod = [&](size_t j) {
+        DevDesc d = descs[j];
+        d.prod = pair_mul[j] >= 0 ? static_cast<U128 *>(muls[pair_mul[j]].c) : nullptr;
+        return d;
+    };
     size_t i0 = 0;
     while (i0 < ntasks) {
         const size_t ti = order[i0];
         const int L = tasks[ti].log2n;
         const TuneChoice c = choice[L];
         if (c.np == 1) {
-            const int Glog = 8 + c.elog - L;  // transforms per item
+            // the run of transforms of this size and table
             size_t i1 = i0;
-            while (i1 < ntasks && tasks[order[i1]].log2n == L && tasks[order[i1]].tw == tasks[ti].tw &&
-                   (i1 - i0) < ((size_t)1 << Glog))
-                i1++;
+            while (i1 < ntasks && tasks[order[i1]].log2n == L && tasks[order[i1]].tw == tasks[ti].tw) i1++;
+            std::vector<size_t> single, firsts;  // unpaired; A of each pair
+            for (size_t j = i0; j < i1; j++) {
+                const size_t t = order[j];
+                if (partner[t] < 0)
+                    single.push_back(t);
+                else if (pair_mul[t] >= 0)
+                    firsts.push_back(t);
+            }
+            const int Glog = 8 + c.elog - L;  // transforms per item
             WorkItem w{};
             w.kind = bntt::ITEM_NTT;
             w.L = (uint8_t)L;
@@ -398,24 +487,42 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
             w.elog = (uint8_t)c.elog;
             w.pass = 0;
             w.np = 1;
-            w.desc = (uint32_t)packed.size();
-            w.nvalid = (uint32_t)(i1 - i0);
             w.first = 0;
-            // transforms back to back in x and in y: one base pointer each
-            bool contig = true;
-            for (size_t j = i0 + 1; j < i1 && contig; j++)
-                contig = descs[order[j]].x == descs[order[i0]].x + ((j - i0) << L) &&
-                         descs[order[j]].y == descs[order[i0]].y + ((j - i0) << L);
-            if (contig) w.flags = bntt::IT_CONTIG;
-            for (size_t j = i0; j < i1; j++) {
-                packed.push_back(descs[order[j]]);
-                out_level[order[j]] = 0;
+            for (size_t s0 = 0; s0 < single.size(); s0 += ((size_t)1 << Glog)) {
+                const size_t s1 = std::min(single.size(), s0 + ((size_t)1 << Glog));
+                w.desc = (uint32_t)packed.size();
+                w.nvalid = (uint32_t)(s1 - s0);
+                w.flags = 0;
+                // transforms back to back in x and in y: one base pointer each
+                bool contig = true;
+                for (size_t j = s0 + 1; j < s1 && contig; j++)
+                    contig = descs[single[j]].x == descs[single[s0]].x + ((j - s0) << L) &&
+                             descs[single[j]].y == descs[single[s0]].y + ((j - s0) << L);
+                if (contig) w.flags = bntt::IT_CONTIG;
+                for (size_t j = s0; j < s1; j++) packed.push_back(descs[single[j]]);
+                ntt_lv[0].push_back(w);
             }
-            ntt_lv[0].push_back(w);
+            // pairs: up to 2^(Glog-1) per item, descriptors A's then B's
+            for (size_t s0 = 0; s0 < firsts.size(); s0 += ((size_t)1 << (Glog - 1))) {
+                const size_t s1 = std::min(firsts.size(), s0 + ((size_t)1 << (Glog - 1)));
+                w.desc = (uint32_t)packed.size();
+                w.nvalid = (uint32_t)(s1 - s0);
+                w.flags = bntt::IT_PAIR | bntt::IT_PROD;
+                for (size_t j = s0; j < s1; j++) packed.push_back(with_prod(firsts[j]));
+                for (size_t j = s0; j < s1; j++) packed.push_back(descs[(size_t)partner[firsts[j]]]);
+                ntt_lv[0].push_back(w);
+            }
+            for (size_t j = i0; j < i1; j++) out_level[order[j]] = 0;
             i0 = i1;
         } else {
+            if (partner[ti] >= 0 && pair_mul[ti] < 0) {  // a pair's B: emitted with its A
+                i0++;
+                continue;
+            }
+            const bool pair = partner[ti] >= 0;
             const uint32_t di = (uint32_t)packed.size();
-            packed.push_back(descs[ti]);
+            packed.push_back(with_prod(ti));
+            if (pair) packed.push_back(descs[(size_t)partner[ti]]);
             int ks[3];
             bntt::pass_split(L, c.np, c.elog, 8 + c.elog, ks);  // valid: checked with the choice
             int S = 0;
@@ -423,24 +530,33 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
                 const int k = ks[pidx];
                 const int Glog = 8 + c.elog - k;
                 const size_t groups = (size_t)1 << (L - k);
-                for (size_t g0 = 0; g0 < groups; g0 += ((size_t)1 << Glog)) {
-                    WorkItem w{};
-                    w.kind = bntt::ITEM_NTT;
-                    w.L = (uint8_t)L;
-                    w.S = (uint8_t)S;
-                    w.k = (uint8_t)k;
-                    w.Glog = (uint8_t)Glog;
-                    w.elog = (uint8_t)c.elog;
-                    w.pass = (uint8_t)pidx;
-                    w.np = (uint8_t)c.np;
-                    w.desc = di;
-                    w.nvalid = (uint32_t)std::min(groups - g0, (size_t)1 << Glog);
-                    w.first = g0;
-                    ntt_lv[start_level[ti] + pidx].push_back(w);
-                }
+                // a pair shares the items of its last pass only (that pass
+                // reads each group contiguously, so half a tile per transform
+                // still moves long runs); earlier passes run each transform
+                // on its own
+                const bool shared = pair && pidx + 1 == c.np;
+                const int plog = Glog - (shared ? 1 : 0);  // groups of one transform per item
+                for (int which = 0; which < (pair && !shared ? 2 : 1); which++)
+                    for (size_t g0 = 0; g0 < groups; g0 += ((size_t)1 << plog)) {
+                        WorkItem w{};
+                        w.kind = bntt::ITEM_NTT;
+                        w.L = (uint8_t)L;
+                        w.S = (uint8_t)S;
+                        w.k = (uint8_t)k;
+                        w.Glog = (uint8_t)Glog;
+                        w.elog = (uint8_t)c.elog;
+                        w.pass = (uint8_t)pidx;
+                        w.np = (uint8_t)c.np;
+                        w.desc = di + (uint32_t)which;
+                        w.nvalid = (uint32_t)std::min(groups - g0, (size_t)1 << plog);
+                        w.first = g0;
+                        if (shared) w.flags = bntt::IT_PAIR | bntt::IT_PROD;
+                        ntt_lv[start_level[ti] + pidx].push_back(w);
+                    }
                 S += k;
             }
             out_level[ti] = start_level[ti] + c.np - 1;
+            if (pair) out_level[(size_t)partner[ti]] = out_level[ti];
             i0++;
         }
     }
@@ -503,6 +619,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         }
         mv[i] = DevMul{static_cast<const U128 *>(t.a), static_cast<const U128 *>(t.b), static_cast<U128 *>(t.c),
                        t.n};
+        if (fused[i]) continue;  // computed by its pair's last pass
         const int lvl = std::max(ready_level(t.a, t.n), ready_level(t.b, t.n));
         for (uint64_t off = 0; off < t.n; off += bntt::kQueueTile) {
             WorkItem w{};
@@ -526,9 +643,12 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         grid_cap = prop.multiProcessorCount;
     }
     // dynamic LDS above 64 KiB must be allowed before occupancy is queried
-    static std::atomic<uint64_t> lds_fast{0}, lds_slow{0};
+    static std::atomic<uint64_t> lds_fast{0}, lds_fast8{0}, lds_slow{0};
     hipError_t attr = bntt_allow_lds(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_FAST>),
                                      (int)bntt::kQueueMaxLds, lds_fast);
+    if (attr == hipSuccess)
+        attr = bntt_allow_lds(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_FAST8>),
+                              (int)bntt::kQueueMaxLds, lds_fast8);
     if (attr == hipSuccess)
         attr = bntt_allow_lds(reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_SLOW>),
                               (int)bntt::kQueueMaxLds, lds_slow);
@@ -556,20 +676,30 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         }
         size_t data_slots = 0;
         std::vector<size_t> needs;
+        bool all8 = true;  // every transform item has a 2048-value tile
         for (const WorkItem &w : lv)
             if (w.kind == bntt::ITEM_NTT) {
                 data_slots = std::max(data_slots, bntt::lds_data_slots(w.k, w.Glog));
-                needs.push_back(bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, w.Glog)));
+                needs.push_back(bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, bntt::item_plog(w))));
+                all8 = all8 && w.elog == 3 && w.k >= 3;
             }
         const size_t budget = (flags & BNTT_PLAN_TW_WIDE) ? bntt::kQueueMaxLds
                               : (flags & BNTT_PLAN_TW_LDS) ? bntt::kQueueLds3
                                                            : 0;
         bntt::TwArea area;
         size_t lds = bntt::plan_tw_area(data_slots, needs.data(), needs.size(), budget, &area);
+        // the code specialised per stage count when the twiddles come from
+        // device memory and every tile is a radix-8 one
+        const int kind = (p->md.pm && budget == 0 && all8) ? bntt::QK_FAST8 : bntt::QK_FAST;
         int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<bntt::QK_FAST>, bntt::kQueueThreads,
-                                                         lds) != hipSuccess || per_cu < 1)
-            per_cu = 1;
+        const hipError_t oe =
+            kind == bntt::QK_FAST8
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<bntt::QK_FAST8>,
+                                                               bntt::kQueueThreads, lds)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bntt::queue_kernel<bntt::QK_FAST>,
+                                                               bntt::kQueueThreads, lds);
+        if (oe != hipSuccess || per_cu < 1) per_cu = 1;
+        p->level_kind.push_back(kind);
         const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), (size_t)grid_cap * per_cu);
         p->level_offset.push_back(all.size());
         p->level_items.push_back(lv.size());
@@ -579,6 +709,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         all.insert(all.end(), lv.begin(), lv.end());
     }
     while (!p->level_items.empty() && p->level_items.back() == 0) {
+        p->level_kind.pop_back();
         p->level_items.pop_back();
         p->level_offset.pop_back();
         p->level_lds.pop_back();
@@ -609,6 +740,13 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         hipDeviceSynchronize() != hipSuccess) {
         free_plan(p);
         return bntt_set_error(BNTT_EHIP, "initialising queue counters failed");
+    }
+    if (p->fixed_tw) {
+        // the tables' digit copies and flags, once
+        if ((rc = convert_tables(p, nullptr)) || hipDeviceSynchronize() != hipSuccess) {
+            free_plan(p);
+            return rc ? rc : bntt_set_error(BNTT_EHIP, "converting the twiddle tables failed");
+        }
     }
     *out = p;
     return BNTT_OK;

@@ -286,10 +286,12 @@ __device__ __forceinline__ void reg_network_slow(U128 (&v)[1 << KP], const TwVie
 }
 
 // One register sub-pass of KP stages over the G groups of 2^k values in LDS.
-// Each thread owns 2^(ELOG-KP) sub-groups.
+// Each thread owns 2^(ELOG-KP) sub-groups.  Group g of the tile is group
+// mbase + (g & pmask) of its transform (pmask < G - 1 when the tile holds the
+// same groups of two transforms, one after the other).
 template <int ELOG, int KP, int MODE, bool PM>
 __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, int S, int r0, uint32_t mbase,
-                                            const Mod &md)
+                                            uint32_t pmask, const Mod &md)
 {
     constexpr int Q = 1 << (ELOG - KP);
     constexpr int R = 1 << KP;
@@ -304,9 +306,10 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
         const int g = sg >> sub_log;
         const int Mp = sg & ((1 << sub_log) - 1);
         if constexpr (MODE != TW_SLOW) {
-            const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);
+            const uint32_t gp = (uint32_t)g & pmask;
+            const uint32_t P0 = (mbase + gp) & ((1u << S) - 1u);
             const uint32_t Mlow = (uint32_t)Mp & ((1u << r0) - 1u);
-            fetch_twiddles<KP, MODE>(t[q], tv, S, r0, (uint32_t)g & (tv.W - 1u), P0, Mlow);
+            fetch_twiddles<KP, MODE>(t[q], tv, S, r0, gp & (tv.W - 1u), P0, Mlow);
         }
         const int src = (g << k) + Mp;
         if (sub_log >= 3) {
@@ -331,7 +334,7 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
         } else if constexpr (MODE != TW_SLOW) {
             reg_network_fast<KP>(v[q], t[q], md);
         } else {
-            const uint32_t P0 = (mbase + (uint32_t)g) & ((1u << S) - 1u);
+            const uint32_t P0 = (mbase + ((uint32_t)g & pmask)) & ((1u << S) - 1u);
             const uint32_t Mlow = (uint32_t)Mp & ((1u << r0) - 1u);
             reg_network_slow<KP, PM>(v[q], tv, S, r0, P0, Mlow, md);
         }
@@ -355,41 +358,41 @@ __device__ __forceinline__ void lds_subpass(U128 *lds, const TwView &tv, int k, 
 // = -1: every sub-pass uses MODE.
 template <int ELOG, int MODE, bool PM, int FALLBACK>
 __device__ __forceinline__ void run_subpass(U128 *lds, const TwView &tv, int k, int S, int r0, int kp, bool lds_tw,
-                                            uint32_t mbase, const Mod &md)
+                                            uint32_t mbase, uint32_t pmask, const Mod &md)
 {
     if constexpr (FALLBACK >= 0) {
         if (!lds_tw) {
             if (kp == ELOG) {
-                lds_subpass<ELOG, ELOG, FALLBACK, PM>(lds, tv, k, S, r0, mbase, md);
+                lds_subpass<ELOG, ELOG, FALLBACK, PM>(lds, tv, k, S, r0, mbase, pmask, md);
             } else if constexpr (ELOG >= 2) {
                 if (kp == 1) {
-                    lds_subpass<ELOG, 1, FALLBACK, PM>(lds, tv, k, S, r0, mbase, md);
+                    lds_subpass<ELOG, 1, FALLBACK, PM>(lds, tv, k, S, r0, mbase, pmask, md);
                 } else if constexpr (ELOG >= 3) {
-                    if (kp == 2) lds_subpass<ELOG, 2, FALLBACK, PM>(lds, tv, k, S, r0, mbase, md);
+                    if (kp == 2) lds_subpass<ELOG, 2, FALLBACK, PM>(lds, tv, k, S, r0, mbase, pmask, md);
                 }
             }
             return;
         }
     }
     if (kp == ELOG) {
-        lds_subpass<ELOG, ELOG, MODE, PM>(lds, tv, k, S, r0, mbase, md);
+        lds_subpass<ELOG, ELOG, MODE, PM>(lds, tv, k, S, r0, mbase, pmask, md);
     } else if constexpr (ELOG >= 2) {
         if (kp == 1) {
-            lds_subpass<ELOG, 1, MODE, PM>(lds, tv, k, S, r0, mbase, md);
+            lds_subpass<ELOG, 1, MODE, PM>(lds, tv, k, S, r0, mbase, pmask, md);
         } else if constexpr (ELOG >= 3) {
-            if (kp == 2) lds_subpass<ELOG, 2, MODE, PM>(lds, tv, k, S, r0, mbase, md);
+            if (kp == 2) lds_subpass<ELOG, 2, MODE, PM>(lds, tv, k, S, r0, mbase, pmask, md);
         }
     }
 }
 
 template <int ELOG, int MODE, bool PM, int FALLBACK = -1>
 __device__ __forceinline__ void run_group_network(U128 *lds, const TwView &tv, int k, int S, uint32_t mbase,
-                                                  const Mod &md, int klds = 32)
+                                                  const Mod &md, int klds = 32, uint32_t pmask = ~0u)
 {
     int r0 = 0;
     while (r0 < k) {
         const int kp = (k - r0) < ELOG ? (k - r0) : ELOG;
-        run_subpass<ELOG, MODE, PM, FALLBACK>(lds, tv, k, S, r0, kp, r0 + kp <= klds, mbase, md);
+        run_subpass<ELOG, MODE, PM, FALLBACK>(lds, tv, k, S, r0, kp, r0 + kp <= klds, mbase, pmask, md);
         r0 += kp;
     }
 }

@@ -126,9 +126,13 @@ typedef struct bntt_ntt_task {
 
 /* One pointwise product c[i] = a[i] * b[i] mod m, i < n (device pointers).
  * When a or b overlaps the y of transforms in the same plan, the product runs
- * after those transforms.  Tasks of a plan are independent: a transform's y
- * may not overlap another transform's x or y, a product's c may not overlap
- * any transform's buffers or another product's operands (BNTT_EINVAL). */
+ * after those transforms.  When a and b are exactly the outputs of two
+ * transforms of the plan of size n and one twiddle table, the two transforms
+ * share their work items and the product is computed in their last pass from
+ * on-chip data (no second read of the outputs).  Tasks of a plan are
+ * independent: a transform's y may not overlap another transform's x or y, a
+ * product's c may not overlap any transform's buffers or another product's
+ * operands (BNTT_EINVAL). */
 typedef struct bntt_mul_task {
     const void *a;
     const void *b;
@@ -166,6 +170,11 @@ typedef struct bntt_batch_plan bntt_batch_plan;
  * workgroups per CU). */
 #define BNTT_PLAN_TW_LDS 2
 #define BNTT_PLAN_TW_WIDE 4
+/* The twiddle tables are not modified after the plan is created: their digit
+ * copies and large-entry checks are made once, at creation, instead of at the
+ * start of every execution.  (Without it an execution picks up tables changed
+ * in place, at the cost of one conversion pass per execution.) */
+#define BNTT_PLAN_TW_FIXED 8
 
 /* Build a plan: device work lists, intermediates for multi-pass sizes, and
  * per-size kernel choices (tune may be NULL for built-in defaults).  The plan

@@ -15,7 +15,11 @@ pytestmark = pytest.mark.gpu
 QN, MUN = C.Q_NTT, C.MU_NTT
 
 
-def build(torch, wl):
+def build(torch, wl, own_tables=False):
+    """Device buffers, transform tasks and the pair products of `wl`.
+    own_tables: every transform reads its own device copy of its size's
+    table, so no two transforms share a table pointer and no product can be
+    fused into its transforms' last pass."""
     dx = to_dev(torch, wl.x)
     dy = torch.empty_like(dx)
     dc = torch.empty((wl.pair_total, 2), dtype=torch.int64, device="cuda")
@@ -23,7 +27,8 @@ def build(torch, wl):
     tasks, muls = [], []
     for i, (l, off) in enumerate(zip(wl.log2n, wl.offsets)):
         n = 1 << int(l)
-        tasks.append((dx[off:off + n], dy[off:off + n], tabs[int(l)]))
+        tw = tabs[int(l)].clone() if own_tables else tabs[int(l)]
+        tasks.append((dx[off:off + n], dy[off:off + n], tw))
     for j, poff in enumerate(wl.pair_offsets):
         n = 1 << int(wl.log2n[2 * j])
         a0, b0 = wl.offsets[2 * j], wl.offsets[2 * j + 1]
@@ -67,13 +72,57 @@ def test_small_uneven_batch(gpu, orc, tw_mode):
     gpu.cuda.synchronize()
     check_all(orc, wl, from_dev(dy), from_dev(dc))
     info = plan.info()
-    # two-pass transforms in two start groups: passes at levels 0-1 and 1-2,
-    # products after each group's last pass (levels 2 and 3)
-    assert info["levels"] == 4 and info["items"] > 0
+    # every product is fused into its pair's last pass: the two passes of the
+    # two-pass sizes are the only levels
+    assert info["levels"] == 2 and info["items"] > 0
     plan.close()
-    # without products every two-pass transform starts at level 0
+    # the same without products
     plan = BatchPlan(tasks, [], QN, MUN, graph=False, tw_mode=tw_mode)
     assert plan.info()["levels"] == 2
+    plan.close()
+
+
+@pytest.mark.parametrize("tw_mode", ["global", "lds"])
+def test_unfused_products(gpu, orc, tw_mode):
+    """Transforms that each read their own table cannot share tiles: their
+    products run as product items after their operands' last pass (two-pass
+    transforms in two start groups: passes at levels 0-1 and 1-2, products
+    after each group's last pass at levels 2 and 3)."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(80, QN, seed=12, lo=1, hi=16)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl, own_tables=True)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=False, tw_mode=tw_mode)
+    plan.execute()
+    gpu.cuda.synchronize()
+    check_all(orc, wl, from_dev(dy), from_dev(dc))
+    assert plan.info()["levels"] == 4
+    plan.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_fixed_tables(gpu, orc, graph):
+    """Plans whose tables are converted once, at creation, over several
+    executions with the inputs refilled between them; a table with an entry
+    >= 2^124 is detected at creation and its transforms (and their fused
+    products) take the literal launch."""
+    from benchntt_amd.batch import BatchPlan
+
+    wl = uneven_pairs(60, QN, seed=19, lo=3, hi=15)
+    sizes = sorted({int(l) for l in wl.log2n})
+    wl.tables[sizes[-2]][5, 1] |= np.uint64(1 << 62)
+    dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
+    plan = BatchPlan(tasks, muls, QN, MUN, graph=graph, fixed_tables=True)
+    stream = gpu.cuda.Stream()
+    for rep in range(3):
+        if rep:
+            wl.x = uniform(wl.total, QN, 300 + rep)
+            dx.copy_(to_dev(gpu, wl.x))
+            gpu.cuda.synchronize()
+        with gpu.cuda.stream(stream):
+            plan.execute()
+        stream.synchronize()
+        check_all(orc, wl, from_dev(dy), from_dev(dc))
     plan.close()
 
 
