@@ -275,7 +275,10 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
     p->md = bntt_make_mod(m, mu);
     p->use_graph = (flags & BNTT_PLAN_GRAPH) != 0;
     p->fixed_tw = (flags & BNTT_PLAN_TW_FIXED) != 0;
-    if (hipEventCreateWithFlags(&p->done, hipEventDisableTiming) != hipSuccess) {
+    // the event only orders executions on one device, which the end of each
+    // kernel already releases at device scope: no system-scope fence (that
+    // would write back the caches at every execution)
+    if (hipEventCreateWithFlags(&p->done, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess) {
         free_plan(p);
         return bntt_set_error(BNTT_EHIP, "hipEventCreateWithFlags");
     }
@@ -691,6 +694,7 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
         // the code specialised per stage count when the twiddles come from
         // device memory and every tile is a radix-8 one
         const int kind = (p->md.pm && budget == 0 && all8) ? bntt::QK_FAST8 : bntt::QK_FAST;
+
         int per_cu = 0;
         const hipError_t oe =
             kind == bntt::QK_FAST8
@@ -700,7 +704,15 @@ int bntt_batch_plan_create(bntt_batch_plan **out, const bntt_ntt_task *tasks, si
                                                                bntt::kQueueThreads, lds);
         if (oe != hipSuccess || per_cu < 1) per_cu = 1;
         p->level_kind.push_back(kind);
-        const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), (size_t)grid_cap * per_cu);
+        // Items of equal work gain nothing from the dynamic draw: then the
+        // level gets a workgroup per item (no counter atomics), otherwise a
+        // persistent grid of the resident capacity
+        bool equal = !lv.empty();
+        for (const WorkItem &w : lv)
+            equal = equal && w.kind == bntt::ITEM_NTT && w.k == lv[0].k && w.L == lv[0].L && w.Glog == lv[0].Glog &&
+                    w.elog == lv[0].elog && w.np == lv[0].np && w.flags == lv[0].flags;
+        const size_t cap = equal ? (size_t)0x7fffffff : (size_t)grid_cap * per_cu;
+        const int grid = (int)std::min<size_t>(std::max<size_t>(lv.size(), 1), cap);
         p->level_offset.push_back(all.size());
         p->level_items.push_back(lv.size());
         p->level_lds.push_back(lds);

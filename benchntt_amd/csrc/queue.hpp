@@ -903,10 +903,14 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
         if (punt.idx) {
             // one read per workgroup, shared with its threads, so that they
             // all see the same count
+            // (a zero count needs no clearing: then no workgroup counts itself;
+            // a nonzero one is read as such by every workgroup, since it is
+            // cleared only after all have read it)
             if (threadIdx.x == 0) {
                 s_ticket = __hip_atomic_load(punt.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (__hip_atomic_fetch_add(&ctl->readers, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
-                    gridDim.x - 1) {
+                if (s_ticket != 0 &&
+                    __hip_atomic_fetch_add(&ctl->readers, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) ==
+                        gridDim.x - 1) {
                     __hip_atomic_store(punt.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     __hip_atomic_store(&ctl->readers, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
@@ -916,7 +920,8 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
             __syncthreads();
         }
     // Workgroup b starts with item b; after each item it draws from the
-    // counter, draw c giving item gridDim + c.  A device-wide atomic is
+    // counter, draw c giving item gridDim + c (unless the grid has a
+    // workgroup per item).  A device-wide atomic is
     // performed beyond the XCDs' L2 caches, so the first round draws nothing.
     // Every processed item is followed by exactly one draw, so the draws are
     // 0 .. nitems-1: the workgroup drawing nitems-1 is the last to touch the
@@ -933,7 +938,12 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
         } else if constexpr (!MONT) {
             done = do_mul_item<KIND>(it, muls, md);
         }
-        if (threadIdx.x == 0) {
+        if (threadIdx.x == 0 && gridDim.x >= nitems) {
+            // one workgroup per item (a level of equal items): no draws
+            if constexpr (qk_fast(KIND))
+                if (!done) punt.idx[atomicAdd(punt.count, 1u)] = t;
+            s_ticket = nitems;
+        } else if (threadIdx.x == 0) {
             if constexpr (qk_fast(KIND))
                 if (!done) punt.idx[atomicAdd(punt.count, 1u)] = t;
             const uint32_t c = atomicAdd(&ctl->ticket, 1u);
