@@ -64,15 +64,16 @@ def load_traffic(name: str):
 
 
 def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 1.5):
-    """The reference's axpy on this host's cores over the same 2^24 vectors
-    (1024-element calls, as the reference kernel is written): passes for about
-    `budget_s` of wall time, which on 16 threads is 10-30 s of CPU work."""
+    """The reference's axpy on every core this process may use, over the same
+    2^24 vectors (1024-element calls, as the reference kernel is written; the
+    reference's own build for the host's ISA, AVX-512 / AVX2 / scalar):
+    passes for about `budget_s` of wall time, 10-30 s of CPU work on 16 cores."""
     from oracle import ref
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads = ref.cpu_threads()
     if ref.available("ref_blas_scalar.so"):
-        isa = ref.best_isa()
-        ops = ref.BlasOperands(a_np, b_np, isa)
+        isa, fl = ref.best_isa(), ref.flavour()
+        ops = ref.BlasOperands(a_np, b_np, isa, fl=fl)
         n = a_np.shape[0]
         ref.blas("axpy", ops, q, mu, s, threads=threads)  # warm
         t_single = ref.blas("axpy", ops, q, mu, s, threads=1)
@@ -82,15 +83,17 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 1.5):
             tot += ref.blas("axpy", ops, q, mu, s, threads=threads)
             passes += 1
         gbs = n * BYTES_PER_ELEM * passes / tot / 1e9
+        host = ref.host_info()
         return {
             "value": round(gbs, 3),
             "unit": "GB/s",
             "cores": threads,
             "kind": "reference",
             "sample": (f"reference blas_{isa}.c axpy over the 2^24-element vectors in 1024-element calls, "
-                       f"{passes} passes, {tot * threads:.0f} s of CPU time, on {threads} threads "
-                       f"({ref.cpu_model()}); single-thread "
-                       f"{n * BYTES_PER_ELEM / t_single / 1e9:.3f} GB/s"),
+                       f"{passes} passes, {tot * threads:.0f} s of CPU time, on {threads} threads (every core "
+                       f"this process may use: nproc {host['nproc']}, affinity {host['affinity']}); "
+                       f"single-thread {n * BYTES_PER_ELEM / t_single / 1e9:.3f} GB/s"),
+            "host": host,
         }
     import oracle
 
@@ -103,12 +106,16 @@ def cpu_baseline(a_np, b_np, q, mu, s, budget_s: float = 1.5):
             "sample": "oracle axpy (C restatement) over the first 2^20 elements, 1 thread"}
 
 
-# VALU instructions per butterfly of the radix-8 register network of the NTT
-# kernels (reg_network_fast<3>: 12 butterflies in 1197 VALU instructions on
-# the path every wave takes unless a lane's product may reach q, counted in
-# the gfx950 ISA of a kernel that loads 8 values and 7 twiddles, runs the
-# network and stores; the network's product is 20 v_mad_u64_u32)
-VALU_PER_BUTTERFLY = 100
+def load_valu_counts() -> dict:
+    """VALU instructions per butterfly of the NTT kernels, per workload, from
+    rocprofv3 SQ counters (SQ_INSTS_VALU x 64 lanes / butterflies; see
+    profiles/ntt_valu_counts.json and the command that produced it)."""
+    path = os.path.join(ROOT, "profiles", "ntt_valu_counts.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("workloads", {})
+    except (OSError, ValueError):
+        return {}
 
 
 def valu_rates() -> dict:
@@ -128,45 +135,170 @@ def valu_rates() -> dict:
     return out
 
 
-def ntt_roofline(gbfly_per_s: float, rates: dict, what: str) -> dict:
-    peak = rates["v_mad_u64_u32"]
-    ach = gbfly_per_s * 1e9 * VALU_PER_BUTTERFLY
-    return {"bound": "int-valu", "kernel": what, "achieved": round(ach / 1e12, 2), "peak": round(peak / 1e12, 2),
-            "unit": "T lane-ops/s", "frac": round(ach / peak, 4), "valu_per_butterfly": VALU_PER_BUTTERFLY,
-            "nominal_peak": round(rates["nominal"] / 1e12, 2)}
+def ntt_roofline(gbfly_per_s: float, rates: dict, what: str, key: str) -> dict:
+    """INT-VALU roofline of an NTT workload: achieved lane-ops/s = butterflies/s
+    x the VALU lane-instructions per butterfly counted for that workload
+    (profiles/ntt_valu_counts.json), against the nominal issue rate of one
+    VALU lane-instruction per lane per clock (CUs x 64 x clock); the measured
+    v_mad_u64_u32 issue rate, the product's own instruction, is given beside."""
+    counts = load_valu_counts().get(key)
+    nominal = rates["nominal"]
+    out = {"bound": "int-valu", "kernel": what, "unit": "T lane-ops/s", "peak": round(nominal / 1e12, 2),
+           "peak_v_mad_u64_u32_measured": round(rates["v_mad_u64_u32"] / 1e12, 2)}
+    if not counts:
+        out.update({"achieved": None, "frac": None, "valu_per_butterfly": None,
+                    "note": f"no SQ counts for {key} in profiles/ntt_valu_counts.json"})
+        return out
+    vpb = counts["valu_per_butterfly"]
+    ach = gbfly_per_s * 1e9 * vpb
+    out.update({"achieved": round(ach / 1e12, 2), "frac": round(ach / nominal, 4),
+                "frac_of_v_mad_rate": round(ach / rates["v_mad_u64_u32"], 4), "valu_per_butterfly": vpb,
+                "valu_per_butterfly_source": counts.get("source", "profiles/ntt_valu_counts.json")})
+    return out
 
 
 def cpu_threads() -> int:
-    return min(os.cpu_count() or 1, 16)
-
-
-def ntt_cpu_rows(np, x, tw, qn, mun) -> dict:
-    """The reference's ntt() at N = 1024 on this host: one core, 50 timed
-    calls after 51 warm-up calls (the reference times 100 calls and averages
-    the last 50, ntt_scalar.c:670-683), scalar and the widest AVX build; then
-    the uniform batch rate with one transform stream per thread."""
-    from concurrent.futures import ThreadPoolExecutor
-
     from oracle import ref
 
-    if not ref.available("ref_ntt_scalar_L10.so"):
-        return {}
-    out = {"ntt_cpu_model": ref.cpu_model()}
-    isas = ["scalar"] + ([ref.best_isa()] if ref.best_isa() != "scalar" else [])
-    for isa in isas:
-        ref.ntt(x, tw, qn, mun, isa=isa, iters=50)
-        _, ns = ref.ntt(x, tw, qn, mun, isa=isa, iters=50)
-        out[f"ntt_1024_cpu_{isa}_ns_per_butterfly_1core"] = round(ns / 5120, 3)
-    isa = isas[-1]
-    T = cpu_threads()
-    with ThreadPoolExecutor(T) as ex:
-        ns = list(ex.map(lambda _: ref.ntt(x, tw, qn, mun, isa=isa, iters=256)[1], range(T)))
-    out[f"ntt_uniform_1024_cpu_{isa}_NTTs_per_s"] = round(T / (float(np.mean(ns)) * 1e-9), 1)
-    out["ntt_cpu_threads"] = T
-    log(f"cpu ntt 1024: " + ", ".join(f"{k.split('_')[3]} {v} ns/butterfly" for k, v in out.items()
-                                      if k.endswith("1core")) + f"; {T} threads {isa}: "
-        f"{out[f'ntt_uniform_1024_cpu_{isa}_NTTs_per_s']:.0f} NTT/s")
+    return ref.cpu_threads()
+
+
+def ntt_sweep(torch, np, qn, mun, cpu: bool = True) -> dict:
+    """The reference's own NTT measurement (benchmark/run_ntt.sh:13, N = 2^8 ..
+    2^20; ntt_scalar.c:670-683,719: ns per butterfly of one ntt() call, mean of
+    the last 50 of 100 calls): the plain one-transform GPU path on device
+    arrays (bntt_ntt_fwd_dev, count 1, back-to-back calls timed with HIP
+    events, and the same calls replayed from a captured graph, which leaves
+    out the host's cost of each call) beside the reference's scalar, AVX2 and AVX-512 ntt() on one host
+    core (its AMD build where the host has the ISA).  Large N on the CPU take
+    fewer calls (still the mean of the last half), so the sweep stays within
+    a few seconds.  Seeded uniform inputs and a seeded table per N."""
+    import ctypes
+
+    from benchntt_amd import _lib
+    from benchntt_amd._lib import U128
+    from benchntt_amd.residues import uniform
+
+    lib = _lib.load()
+    mq, mm = U128.of(qn), U128.of(mun)
+    dev = torch.device("cuda:0")
+    s = torch.cuda.Stream()
+    rows = []
+    have_ref = False
+    if cpu:
+        from oracle import ref
+
+        have_ref = ref.available("ref_ntt_scalar_L8.so")
+        isas = ["scalar", "avx2"] + (["avx512"] if ref.best_isa() == "avx512" else [])
+        fl = ref.flavour()
+    for L in range(8, 21):
+        n = 1 << L
+        bfly = n * L // 2
+        x_np, tw_np = uniform(n, qn, 3 + L), uniform(n, qn, 40 + L)
+        x = torch.from_numpy(x_np.view(np.int64)).to(dev)
+        tw = torch.from_numpy(tw_np.view(np.int64)).to(dev)
+        y = torch.empty_like(x)
+        need = lib.bntt_ntt_tmp_bytes(L, 1) // 16
+        tmp = torch.empty((max(need, 1), 2), dtype=torch.int64, device=dev)
+        args = (y.data_ptr(), x.data_ptr(), tw.data_ptr(), L, 1, ctypes.byref(mq), ctypes.byref(mm),
+                tmp.data_ptr() if need else None, ctypes.c_void_p(s.cuda_stream))
+        calls = 100 if L <= 16 else 20
+        for _ in range(calls // 2):
+            _lib.check(lib.bntt_ntt_fwd_dev(*args))
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(s)
+        for _ in range(calls // 2):
+            _lib.check(lib.bntt_ntt_fwd_dev(*args))
+        ev1.record(s)
+        torch.cuda.synchronize()
+        us = ev0.elapsed_time(ev1) / (calls // 2) * 1e3
+        # the same calls replayed from a captured graph: the device's time per
+        # transform without the host's cost of issuing each call
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            for _ in range(20):
+                _lib.check(lib.bntt_ntt_fwd_dev(*args[:-1], ctypes.c_void_p(s.cuda_stream)))
+        g.replay()
+        torch.cuda.synchronize()
+        ev0.record(s)
+        with torch.cuda.stream(s):
+            for _ in range(5):
+                g.replay()
+        ev1.record(s)
+        torch.cuda.synchronize()
+        gus = ev0.elapsed_time(ev1) / 100 * 1e3
+        del g
+        row = {"log2n": L, "gpu_plain_us": round(us, 2), "gpu_plain_ns_per_butterfly": round(us * 1e3 / bfly, 4),
+               "gpu_plain_graph_us": round(gus, 2), "gpu_plain_graph_ns_per_butterfly": round(gus * 1e3 / bfly, 4)}
+        if have_ref:
+            for isa in isas:
+                _, ns1 = ref.ntt(x_np, tw_np, qn, mun, isa=isa, iters=2, fl=fl)
+                iters = int(min(100, max(4, 0.15e9 / max(ns1, 1.0))))
+                iters -= iters % 2
+                _, ns = ref.ntt(x_np, tw_np, qn, mun, isa=isa, iters=iters, fl=fl)
+                row[f"cpu_{isa}_ns_per_butterfly"] = round(ns / bfly, 4)
+                row[f"cpu_{isa}_calls"] = iters
+        rows.append(row)
+        log(f"sweep N=2^{L}: GPU plain {us:.2f} us ({row['gpu_plain_ns_per_butterfly']} ns/butterfly), "
+            f"{gus:.2f} us from a graph"
+            + "".join(f", {k.split('_')[1]} {v} ns/butterfly" for k, v in row.items()
+                      if k.startswith("cpu_") and k.endswith("butterfly")))
+    out = {"ntt_sweep": rows}
+    if have_ref:
+        out["ntt_sweep_cpu"] = {"cores": 1, "flavour": fl, "compiler_flags": ref.FLAVOUR_FLAGS[fl],
+                                "model": ref.host_info()["model"]}
     return out
+
+
+def blas_1024_rows(torch, np, bn, q, mu, s, cpu: bool = True) -> dict:
+    """The reference's BLAS measurement at its own size (blas_scalar.c:660-721,
+    777-779: ns per element of one 1024-element call, mean of the last 50 of
+    100 calls; the AVX-512 add and sub take the last 500 of 1000): each op on
+    the GPU through device arrays (back-to-back launches, HIP events) and
+    through the reference's call shape (host arrays in and out, one call at a
+    time, from Python), beside the reference's scalar, AVX2 and AVX-512 kernels
+    on one host core.  Operands: seeded uniform residues mod q_B."""
+    from benchntt_amd.residues import uniform
+
+    n = 1024
+    a_np, b_np = uniform(n, q, 11), uniform(n, q, 12)
+    dev = torch.device("cuda:0")
+    da = torch.from_numpy(a_np.view(np.int64)).to(dev)
+    db = torch.from_numpy(b_np.view(np.int64)).to(dev)
+    dc = torch.empty_like(da)
+    ch = np.empty_like(a_np)
+    st = torch.cuda.current_stream()
+    fns = {
+        "vvadd": (lambda: bn.vector_add(dc, da, db, q), lambda: bn.vector_add(ch, a_np, b_np, q)),
+        "vvsub": (lambda: bn.vector_sub(dc, da, db, q), lambda: bn.vector_sub(ch, a_np, b_np, q)),
+        "vvmul": (lambda: bn.vector_mul(dc, da, db, q, mu), lambda: bn.vector_mul(ch, a_np, b_np, q, mu)),
+        "axpy": (lambda: bn.axpy(dc, da, db, s, q, mu), lambda: bn.axpy(ch, a_np, b_np, s, q, mu)),
+    }
+    out = {}
+    ref_ops = {}
+    if cpu:
+        from oracle import ref
+
+        if ref.available("ref_blas_scalar.so"):
+            fl = ref.flavour()
+            isas = ["scalar", "avx2"] + (["avx512"] if ref.best_isa() == "avx512" else [])
+            ref_ops = {isa: ref.BlasOperands(a_np, b_np, isa, fl=fl) for isa in isas}
+    for op, (fdev, fhost) in fns.items():
+        ms = time_kernel(torch, fdev, 50, 50, st)
+        row = {"gpu_device_ns_per_element": round(ms * 1e6 / n, 3)}
+        for _ in range(20):
+            fhost()
+        t0 = time.perf_counter()
+        for _ in range(100):
+            fhost()
+        row["gpu_host_entry_ns_per_element"] = round((time.perf_counter() - t0) / 100 * 1e9 / n, 3)
+        for isa, ops in ref_ops.items():
+            iters = 1000 if (isa == "avx512" and op in ("vvadd", "vvsub")) else 100
+            row[f"cpu_{isa}_ns_per_element"] = round(ref.blas_ns_per_element(op, ops, q, mu, s, iters), 3)
+        out[op] = row
+        log(f"blas {op} n=1024: " + ", ".join(f"{k} {v}" for k, v in row.items()))
+    return {"blas_1024": out}
 
 
 def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: bool = False) -> dict:
@@ -187,8 +319,6 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     out["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
     out["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
     out["ntt_plain_1024_butterflies_per_s"] = round(5120 / (ms1 * 1e-3), 1)
-    # INT-op/s = butterflies/s x the network's VALU lane-instructions per butterfly
-    out["ntt_plain_1024_int_ops_per_s"] = round(5120 / (ms1 * 1e-3) * VALU_PER_BUTTERFLY, 1)
     # the reference's call shape: host arrays in, host array out, one call at a
     # time (PCIe and synchronisation included; the library's zero-copy path)
     yh = np.empty_like(x_np)
@@ -228,33 +358,56 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
         log(f"uniform {cnt}x{n}: persistent {msp:.4f} ms ({out[f'{key}_NTTs_per_s']:.0f} NTT/s)"
             + (f", one launch {msd:.4f} ms, per-transform launches {mse:.2f} ms" if L == 10 else ""))
     if cpu:
-        out.update(ntt_cpu_rows(np, x_np, tw_np, qn, mun))
+        out.update(uniform_cpu(np, x_np, tw_np, qn, mun))
     return out
 
 
-def uneven_cpu(np, wl, qn, mun) -> dict:
-    """The same 20,000 transforms through the reference's scalar ntt() (one
-    build per size), spread over the host's threads: NTTs/s = transforms /
-    (sum of the per-transform times, measured concurrently, / threads)."""
+def uniform_cpu(np, x, tw, qn, mun) -> dict:
+    """The uniform batch on the CPU: the reference's fastest ntt() build for
+    the host at N = 1024, one transform stream per core, every core this
+    process may use."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import ref
 
-    if not all(ref.available(f"ref_ntt_scalar_L{l}.so") for l in set(int(v) for v in wl.log2n)):
+    if not ref.available("ref_ntt_scalar_L10.so"):
+        return {}
+    isa, fl = ref.best_isa(), ref.flavour()
+    T = cpu_threads()
+    ref.ntt(x, tw, qn, mun, isa=isa, iters=4, fl=fl)
+    with ThreadPoolExecutor(T) as ex:
+        ns = list(ex.map(lambda _: ref.ntt(x, tw, qn, mun, isa=isa, iters=256, fl=fl)[1], range(T)))
+    rate = T / (float(np.mean(ns)) * 1e-9)
+    log(f"cpu uniform 1024 ({isa}, {T} threads): {rate:.0f} NTT/s")
+    return {f"ntt_uniform_1024_cpu_{isa}_NTTs_per_s": round(rate, 1), "ntt_cpu_threads": T}
+
+
+def uneven_cpu(np, wl, qn, mun) -> dict:
+    """The same 20,000 transforms through the reference's fastest ntt() build
+    for the host (AVX-512 where available, one build per size), spread over
+    every core this process may use: NTTs/s = transforms / (sum of the
+    per-transform times, measured concurrently, / threads)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from oracle import ref
+
+    isa, fl = ref.best_isa(), ref.flavour()
+    sfx = "_amd" if fl == "amd" else ""
+    if not all(ref.available(f"ref_ntt_{isa}{sfx}_L{l}.so") for l in set(int(v) for v in wl.log2n)):
         return {}
     T = cpu_threads()
 
     def one(i):
         l, off = int(wl.log2n[i]), int(wl.offsets[i])
-        return ref.ntt(wl.x[off:off + (1 << l)], wl.tables[l], qn, mun, isa="scalar", iters=1)[1]
+        return ref.ntt(wl.x[off:off + (1 << l)], wl.tables[l], qn, mun, isa=isa, iters=2, fl=fl)[1]
 
     t0 = time.perf_counter()
     with ThreadPoolExecutor(T) as ex:
         ns = float(sum(ex.map(one, range(len(wl.log2n)))))
     wall = time.perf_counter() - t0
     rate = len(wl.log2n) / (ns * 1e-9 / T)
-    log(f"cpu uneven 20000 (reference scalar, {T} threads): {rate:.0f} NTT/s, wall {wall:.1f} s")
-    return {"uneven_cpu_scalar_NTTs_per_s": round(rate, 1), "uneven_cpu_threads": T}
+    log(f"cpu uneven 20000 (reference {isa}, {T} threads): {rate:.0f} NTT/s, wall {wall:.1f} s")
+    return {f"uneven_cpu_{isa}_NTTs_per_s": round(rate, 1), "uneven_cpu_threads": T}
 
 
 def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: bool = False) -> dict:
@@ -310,7 +463,7 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     return out
 
 
-def serving(torch, np, n_req: int = 16384, cpu: bool = True) -> dict:
+def serving(torch, np, n_req: int = 16384) -> dict:
     """Negacyclic polynomial-multiply requests mod q_116 (forward NTT x2,
     pointwise product, inverse NTT) with N uniform on 2^10..2^16, served with
     continuous batching under a 2 GiB HBM budget on 4 streams.  First a closed
@@ -369,6 +522,30 @@ def serving(torch, np, n_req: int = 16384, cpu: bool = True) -> dict:
     at = sweep[2]
     out["serving_p50_us_at_0.7_saturation"] = at["p50_us"]
     out["serving_p99_us_at_0.7_saturation"] = at["p99_us"]
+    # admission under a budget that binds: the 0.7 x saturation trace again
+    # with a 128 MiB arena (about 100 requests of the mean size), results
+    # compared on the device with the unconstrained run's (outside the timing)
+    ref_c = Cc.clone()
+    tight = 128 << 20
+    srv_t = PolymulServer(q, hbm_budget=tight, max_inflight=4, max_batch=64, min_log2n=10, max_log2n=16)
+    srv_t.multiply(reqs[:4])
+    Cc.zero_()
+    gaps = np.random.default_rng(700).exponential(1e9 / (0.7 * sat), size=n_req)
+    arrival = np.cumsum(gaps).astype(np.uint64)
+    done, st = srv_t.run_trace(reqs, arrival)
+    lat = (done.astype(np.float64) - arrival.astype(np.float64)) * 1e-3
+    span = float(done.max()) * 1e-9
+    same = bool(torch.equal(Cc, ref_c))
+    out["serving_budget_bound"] = {
+        "hbm_budget_bytes": tight, "offered_per_s": round(0.7 * sat, 1), "achieved_per_s": round(n_req / span, 1),
+        "p50_us": round(float(np.percentile(lat, 50)), 1), "p99_us": round(float(np.percentile(lat, 99)), 1),
+        "admission_waits": int(st["admission_waits"]), "max_inflight_requests": int(st["max_inflight_requests"]),
+        "batches": int(st["batches"]), "results_equal_unconstrained": same}
+    log(f"serving with a {tight >> 20} MiB budget at 0.7 x saturation: achieved {n_req / span:.0f}/s, "
+        f"p50 {out['serving_budget_bound']['p50_us']} us, p99 {out['serving_budget_bound']['p99_us']} us, "
+        f"{st['admission_waits']} admission waits, results equal: {same}")
+    srv_t.close()
+    del ref_c
     # no batching: one request per bntt_pm_multiply call
     k = 256
     t0 = time.perf_counter()
@@ -379,24 +556,6 @@ def serving(torch, np, n_req: int = 16384, cpu: bool = True) -> dict:
     out["serving_batched_vs_unbatched"] = round(sat * one, 2)
     log(f"serving one request per call: {one * 1e6:.1f} us mean -> {1 / one:.0f} req/s")
     srv.close()
-    if cpu:
-        import oracle
-
-        orc = oracle.load()
-        n = 1 << 12
-        from benchntt_amd.residues import uniform
-
-        a, b = uniform(n, q, 1), uniform(n, q, 2)
-        t0, cnt = time.perf_counter(), 0
-        while time.perf_counter() - t0 < 3.0:
-            orc.polymul_ntt(a, b, q)
-            cnt += 1
-        dt = (time.perf_counter() - t0) / cnt
-        out["serving_cpu_baseline"] = {
-            "value": round(1 / dt, 2), "unit": "requests/s at N=2^12", "cores": 1, "kind": "port",
-            "sample": f"{cnt} negacyclic products of N=2^12 mod q_116 through the exact C restatement "
-                      "(bitwise 256-by-128 reduction, table set-up included), about 3 s"}
-        log(f"serving cpu port: {1 / dt:.2f} req/s at N=2^12")
     return out
 
 
@@ -409,11 +568,13 @@ def main() -> int:
     ap.add_argument("--skip-serving", action="store_true", help="leave out the serving trace")
     ap.add_argument("--skip-per-launch", action="store_true",
                     help="skip the one-launch-per-transform comparisons (keeps profiler output small)")
-    ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven", "serving"], default="all",
+    ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven", "serving", "sweep", "blas1024"],
+                    default="all",
                     help="time one workload only and print no JSON line (for profiling runs): "
                          "axpy 2^24; copy (a 256 MiB device copy, the traffic calibration kernel); "
                          "ntt-uniform (8192 x N=1024); uneven (20,000-transform batch); "
-                         "serving (polynomial-multiply trace)")
+                         "serving (polynomial-multiply trace); sweep (plain NTT N = 2^8..2^20 beside the "
+                         "reference's CPU ntt()); blas1024 (the reference's n = 1024 BLAS protocol)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -436,7 +597,13 @@ def main() -> int:
     log(f"device {props.name} ({getattr(props, 'gcnArchName', '?')}), {props.multi_processor_count} CUs")
 
     if args.only == "serving":
-        serving(torch, np, cpu=False)
+        serving(torch, np)
+        return 0
+    if args.only == "sweep":
+        ntt_sweep(torch, np, C.Q_NTT, C.MU_NTT, cpu=not args.no_cpu_baseline)
+        return 0
+    if args.only == "blas1024":
+        blas_1024_rows(torch, np, bn, C.Q_BLAS, C.MU_BLAS, C.AXPY_SCALAR, cpu=not args.no_cpu_baseline)
         return 0
     if args.only in ("ntt-uniform", "uneven"):
         qn, mun = C.Q_NTT, C.MU_NTT
@@ -488,17 +655,23 @@ def main() -> int:
         del src, dst
 
         cpu = not args.no_cpu_baseline
+        extras.update(blas_1024_rows(torch, np, bn, q, mu, s, cpu))
+        extras.update(ntt_sweep(torch, np, C.Q_NTT, C.MU_NTT, cpu))
         extras.update(uniform_rows(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
         extras.update(uneven_batch(torch, np, bn, C.Q_NTT, C.MU_NTT, stream, not args.skip_per_launch, cpu))
         if not args.skip_serving:
-            extras.update(serving(torch, np, cpu=not args.no_cpu_baseline))
+            extras.update(serving(torch, np))
         rates = valu_rates()
         extras["valu_issue_rates_T_lane_ops_per_s"] = {k: round(v / 1e12, 3) for k, v in rates.items()}
         extras["ntt_roofline_uniform_8192x1024"] = ntt_roofline(
+            extras["ntt_uniform_8192x1024_NTTs_per_s"] * 5120 / 1e9, rates,
+            "batch plan (queue_kernel<QK_FAST8>), 8192 x N=1024, whole execution", "uniform_8192x1024_plan")
+        extras["ntt_roofline_uniform_8192x1024_single_launch"] = ntt_roofline(
             extras["ntt_uniform_8192x1024_single_launch_NTTs_per_s"] * 5120 / 1e9, rates,
-            "single-launch kernel, 8192 x N=1024 (ntt_pass_kernel<3>)")
+            "single-launch kernel (ntt_pass_kernel<3>), 8192 x N=1024", "uniform_8192x1024_single_launch")
         extras["ntt_roofline_uneven_20000"] = ntt_roofline(
-            extras["uneven_Gbutterflies_per_s"], rates, "persistent work-queue kernel, 20,000 transforms 2^4..2^16")
+            extras["uneven_Gbutterflies_per_s"], rates,
+            "batch plan (queue_kernel<QK_FAST8>), 20,000 transforms 2^4..2^16, whole execution", "uneven_20000")
 
     traffic = load_traffic("axpy_2_24")
     roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -64,7 +64,7 @@ u128 barrett_mu_251(u128 m)
     return q;
 }
 
-Mod make_mod(const bntt_u128 *m, const bntt_u128 *mu)
+Mod make_mod_uncached(const bntt_u128 *m, const bntt_u128 *mu)
 {
     Mod md{};
     md.m = bntt::make_u128(m->lo, m->hi);
@@ -81,6 +81,24 @@ Mod make_mod(const bntt_u128 *m, const bntt_u128 *mu)
         }
     }
     return md;
+}
+
+// The descriptor of the calling thread's last modulus (the check of mu costs
+// a 252-step division, which would otherwise be paid on every call).
+Mod make_mod(const bntt_u128 *m, const bntt_u128 *mu)
+{
+    thread_local bntt_u128 last_m{~0ull, ~0ull}, last_mu{~0ull, ~0ull};
+    thread_local bool last_has_mu = false;
+    thread_local Mod last{};
+    const bool has_mu = mu != nullptr;
+    if (m->lo == last_m.lo && m->hi == last_m.hi && has_mu == last_has_mu &&
+        (!has_mu || (mu->lo == last_mu.lo && mu->hi == last_mu.hi)))
+        return last;
+    last = make_mod_uncached(m, mu);
+    last_m = *m;
+    last_mu = has_mu ? *mu : bntt_u128{0, 0};
+    last_has_mu = has_mu;
+    return last;
 }
 
 // Intermediates of bntt_ntt_fwd_dev_noalloc: a fixed array in device memory,

@@ -15,9 +15,19 @@
 # (benchmark/run_ntt.sh:14).  The same substitution is applied here on a pipe
 # into the compiler, so no edited copy of any file is written anywhere.
 #
-# Flags follow the reference Makefile (-O3 -w, Makefile:8-9) with a portable
-# -march in place of -march=native, because the objects are built here and run
-# on another host: x86-64-v3 (AVX2) and x86-64-v4 (AVX-512F/BW/CD/DQ/VL).
+# Two flavours of every object:
+#   portable  gcc -O3 -w with -march=x86-64-v3 (AVX2) / x86-64-v4 (AVX-512F/BW/
+#             CD/DQ/VL) in place of the Makefile's default -march=native
+#             (Makefile:8-9), because the objects are built here and run on
+#             another host; these run anywhere the ISA exists (tests here)
+#   amd       the reference's own AMD configuration, AMD_CC = clang with
+#             AMD_CFLAGS = -march=znver4 -O3 -w (Makefile:4-5), with the ROCm
+#             clang; these are the CPU baseline on hosts with znver4's ISA.
+#             clang 22 makes an incompatible pointer type an error by default;
+#             the AVX2 files pass uint64_t * to _mm256_storeu_si256, which the
+#             clang of the reference's time only warned about (silenced by -w),
+#             so that error is turned back into a warning for them
+# Sizes: L = 4..20 for every ISA (benchmark/run_ntt.sh:13 sweeps 8..20).
 set -euo pipefail
 
 REF=${BNTT_REFERENCE:-/root/reference}
@@ -28,42 +38,56 @@ if [ ! -f "$REF/src/ntt_scalar.c" ]; then
     exit 0
 fi
 mkdir -p "$OUT"
-CC=${CC:-gcc}
-COMMON=(-O3 -w -fPIC -fno-semantic-interposition -shared -I"$REF/src" -Dmain=bntt_ref_unused_main)
+GCC=${CC:-gcc}
+CLANG=${BNTT_REF_CLANG:-/opt/rocm/llvm/bin/clang}
 GLUE="$HERE/ref_glue.c"
 CFG="$REF/src/config.c"
+COMMON=(-O3 -w -fPIC -fno-semantic-interposition -I"$REF/src" -Dmain=bntt_ref_unused_main)
 
-build_blas() {  # variant macro march
-    local src=$1 def=$2 march=$3 out=$4
-    $CC "${COMMON[@]}" -march=$march -D$def "$REF/src/$src" "$CFG" "$GLUE" -o "$OUT/$out"
+# flags of one compiler: clang's pointer-type error back to a warning (above)
+cflags() { if [ "$1" = "$CLANG" ]; then echo "-Wno-error=incompatible-pointer-types"; fi; }
+
+build_blas() {  # cc march file macro out
+    local cc=$1 march=$2 src=$3 def=$4 out=$5
+    $cc "${COMMON[@]}" $(cflags $cc) -shared -march=$march -D$def "$REF/src/$src" "$CFG" "$GLUE" -o "$OUT/$out"
 }
 
-build_ntt() {   # file macro march stages out
-    local src=$1 def=$2 march=$3 stages=$4 out=$5
+build_ntt() {   # cc march file macro stages out
+    local cc=$1 march=$2 src=$3 def=$4 stages=$5 out=$6
     local obj="$OUT/.${out%.so}.o"
     sed "s/^#define stages [0-9]*/#define stages ${stages}/" "$REF/src/$src" |
-        $CC -O3 -w -fPIC -fno-semantic-interposition -c -march=$march -I"$REF/src" -Dmain=bntt_ref_unused_main -x c - -o "$obj"
-    $CC "${COMMON[@]}" -march=$march -D$def -DBNTT_REF_STAGES=$stages "$obj" "$CFG" "$GLUE" -o "$OUT/$out"
+        $cc "${COMMON[@]}" $(cflags $cc) -c -march=$march -x c - -o "$obj"
+    $cc "${COMMON[@]}" $(cflags $cc) -shared -march=$march -D$def -DBNTT_REF_STAGES=$stages "$obj" "$CFG" "$GLUE" \
+        -o "$OUT/$out"
     rm -f "$obj"
 }
 
-build_blas blas_scalar.c REF_BLAS_SCALAR x86-64-v3 ref_blas_scalar.so &
-build_blas blas_avx2.c REF_BLAS_AVX2 x86-64-v3 ref_blas_avx2.so &
-build_blas blas_avx512.c REF_BLAS_AVX512 x86-64-v4 ref_blas_avx512.so &
-wait
-# scalar NTT at every size the reference sweep covers, plus the small fixture sizes
-for L in 4 5 6 7 8 9 10 11 12 13 14 15 16 17 18 19 20; do
-    build_ntt ntt_scalar.c REF_NTT_SCALAR x86-64-v3 $L ref_ntt_scalar_L$L.so &
+flavours="portable"
+if [ -x "$CLANG" ] && "$CLANG" -march=znver4 -E -x c /dev/null -o /dev/null 2>/dev/null; then
+    flavours="portable amd"
+fi
+jobs=0
+throttle() { jobs=$((jobs + 1)); if [ $jobs -ge ${BNTT_REF_JOBS:-8} ]; then wait; jobs=0; fi; }
+for fl in $flavours; do
+    if [ $fl = portable ]; then
+        cc=$GCC; m2=x86-64-v3; m5=x86-64-v4; sfx=""
+    else
+        cc=$CLANG; m2=znver4; m5=znver4; sfx="_amd"
+    fi
+    build_blas $cc $m2 blas_scalar.c REF_BLAS_SCALAR ref_blas_scalar$sfx.so & throttle
+    build_blas $cc $m2 blas_avx2.c REF_BLAS_AVX2 ref_blas_avx2$sfx.so & throttle
+    build_blas $cc $m5 blas_avx512.c REF_BLAS_AVX512 ref_blas_avx512$sfx.so & throttle
+    for L in $(seq 4 20); do
+        build_ntt $cc $m2 ntt_scalar.c REF_NTT_SCALAR $L ref_ntt_scalar${sfx}_L$L.so & throttle
+        build_ntt $cc $m2 ntt_avx2.c REF_NTT_AVX2 $L ref_ntt_avx2${sfx}_L$L.so & throttle
+        build_ntt $cc $m5 ntt_avx512.c REF_NTT_AVX512 $L ref_ntt_avx512${sfx}_L$L.so & throttle
+    done
 done
 wait
-for L in 4 6 8 10 12 14 16; do
-    build_ntt ntt_avx2.c REF_NTT_AVX2 x86-64-v3 $L ref_ntt_avx2_L$L.so &
-    build_ntt ntt_avx512.c REF_NTT_AVX512 x86-64-v4 $L ref_ntt_avx512_L$L.so &
-done
-wait
+want=$((54 * $(echo $flavours | wc -w)))
 n=$(ls "$OUT"/*.so | wc -l)
-if [ "$n" -ne 34 ]; then
-    echo "build_ref: expected 34 objects, found $n" >&2
+if [ "$n" -lt "$want" ]; then
+    echo "build_ref: expected $want objects, found $n" >&2
     exit 1
 fi
-echo "build_ref: built $n objects into $OUT"
+echo "build_ref: built $n objects ($flavours) into $OUT"

@@ -48,6 +48,64 @@ def best_isa() -> str:
     return "scalar"
 
 
+# The ISA extensions -march=znver4 may use (the reference's AMD build,
+# Makefile:4-5): the "amd" objects run only where all are present.
+ZNVER4_FLAGS = {"avx2", "bmi2", "adx", "avx512f", "avx512dq", "avx512cd", "avx512bw", "avx512vl", "avx512ifma",
+                "avx512vbmi", "avx512_vbmi2", "avx512_vnni", "avx512_bitalg", "avx512_vpopcntdq", "avx512_bf16",
+                "gfni", "vaes", "vpclmulqdq"}
+
+# compiler and flags of each flavour (oracle/build_ref.sh)
+FLAVOUR_FLAGS = {
+    "amd": "clang (ROCm 7.2, LLVM 22) -march=znver4 -O3 -w: the reference's AMD configuration, Makefile:4-5",
+    "portable": "gcc 11 -O3 -w -march=x86-64-v3 (scalar, AVX2) / x86-64-v4 (AVX-512)",
+}
+
+
+def flavour() -> str:
+    """'amd' where the host has znver4's ISA and those objects were built,
+    else 'portable'."""
+    if ZNVER4_FLAGS <= cpu_flags() and os.path.exists(os.path.join(REF_DIR, "ref_blas_scalar_amd.so")):
+        return "amd"
+    return "portable"
+
+
+def _sfx(fl: str | None) -> str:
+    return "_amd" if (fl or flavour()) == "amd" else ""
+
+
+def cpu_threads() -> int:
+    """The cores this process may use: its affinity set, capped by a cgroup
+    CPU quota when one is set (cpu.max)."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
+
+
+def host_info() -> dict:
+    """What the CPU baseline ran on: core counts, model and the flags of the
+    reference objects used."""
+    model = cpu_model()
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    fl = cpu_flags()
+    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "threads_used": cpu_threads(),
+            "model": model, "isa": sorted(f for f in fl if f.startswith(("avx", "amx")) or f in ZNVER4_FLAGS),
+            "flavour": flavour(), "compiler_flags": FLAVOUR_FLAGS[flavour()]}
+
+
 def cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
@@ -98,11 +156,15 @@ def _aligned(nwords: int) -> np.ndarray:
     return buf[off:off + nwords]
 
 
-def ntt(x: np.ndarray, tw: np.ndarray, m: int, mu: int, isa: str = "scalar", iters: int = 0):
-    """Run the reference ntt at N = len(tw).  Returns (y, ns_per_call)."""
+def ntt(x: np.ndarray, tw: np.ndarray, m: int, mu: int, isa: str = "scalar", iters: int = 0,
+        fl: str | None = "portable"):
+    """Run the reference ntt at N = len(tw).  Returns (y, ns_per_call): the
+    mean of the last half of `iters` timed calls after one warm call (the
+    reference's protocol, ntt_scalar.c:670-683, with iters = 100).  fl: the
+    build flavour ('portable', 'amd', or None for the host's best)."""
     n = tw.reshape(-1, 2).shape[0]
     log2n = n.bit_length() - 1
-    lib = _lib(f"ref_ntt_{isa}_L{log2n}.so")
+    lib = _lib(f"ref_ntt_{isa}{_sfx(fl)}_L{log2n}.so")
     lib.bntt_ref_ntt.restype = ctypes.c_double
     lib.bntt_ref_ntt.argtypes = [_P, _P, _P, _P, _P, ctypes.c_int]
     mm, mmu = _u128(m), _u128(mu)
@@ -126,7 +188,8 @@ def ntt(x: np.ndarray, tw: np.ndarray, m: int, mu: int, isa: str = "scalar", ite
 class BlasOperands:
     """Operands laid out once for one reference BLAS variant (untimed)."""
 
-    def __init__(self, a: np.ndarray, b: np.ndarray, isa: str):
+    def __init__(self, a: np.ndarray, b: np.ndarray, isa: str, fl: str | None = "portable"):
+        self.lib_name = f"ref_blas_{isa}{_sfx(fl)}.so"
         a = np.asarray(a, dtype=np.uint64).reshape(-1, 2)
         b = np.asarray(b, dtype=np.uint64).reshape(-1, 2)
         n = a.shape[0]
@@ -151,7 +214,7 @@ def blas(op: str, ops: BlasOperands, m: int, mu: int, x: int = 0, threads: int =
     """Run the reference kernel `op` over all 1024-blocks of `ops`, split into
     contiguous block ranges over `threads` threads (ctypes releases the GIL).
     Returns wall-clock seconds."""
-    lib = _lib(f"ref_blas_{ops.isa}.so")
+    lib = _lib(ops.lib_name)
     lib.bntt_ref_blas.restype = ctypes.c_double
     lib.bntt_ref_blas.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_long, _P, _P, _P]
     mm, mmu, xx = _u128(m), _u128(mu), _u128(x)
@@ -176,6 +239,19 @@ def blas(op: str, ops: BlasOperands, m: int, mu: int, x: int = 0, threads: int =
         for th in ths:
             th.join()
     return time.perf_counter() - t0
+
+
+def blas_ns_per_element(op: str, ops: BlasOperands, m: int, mu: int, x: int = 0, iters: int = 100) -> float:
+    """The reference's BLAS protocol on the first 1024 elements of `ops`
+    (blas_scalar.c:660-721): `iters` calls, ns per element of the mean of the
+    last half, on the calling thread."""
+    lib = _lib(ops.lib_name)
+    lib.bntt_ref_blas_repeat.restype = ctypes.c_double
+    lib.bntt_ref_blas_repeat.argtypes = [ctypes.c_int, _P, _P, _P, ctypes.c_int, _P, _P, _P]
+    mm, mmu, xx = _u128(m), _u128(mu), _u128(x)
+    ns = lib.bntt_ref_blas_repeat(BLAS_OPS[op], ops.c.ctypes.data, ops.a.ctypes.data, ops.b.ctypes.data, iters,
+                                  mm.ctypes.data, mmu.ctypes.data, xx.ctypes.data)
+    return ns / 1024
 
 
 def scalar_primitive(name: str, a: int, b: int, m: int, mu: int = 0) -> int:

@@ -37,21 +37,37 @@ static double now_ns(void)
 
 int bntt_ref_log2n(void) { return BNTT_REF_STAGES; }
 
+/* The reference's timing protocols take the mean of the last half of their
+ * calls (ntt_scalar.c:670-683: 100 calls, mean of the last 50; the BLAS
+ * drivers likewise, blas_scalar.c:660-721, with 1000 and the last 500 for the
+ * AVX-512 add and sub, blas_avx512.c:560-590). */
+static double mean_last_half(double *t, int iters)
+{
+    double s = 0.0;
+    for (int i = iters / 2; i < iters; i++) s += t[i];
+    return s / (iters - iters / 2);
+}
+
 /* ------------------------------------------------------------------ NTT */
 #if defined(REF_NTT_SCALAR)
 void ntt(u128 *Y, u128 *X, u128 modulus, u128 mu, u128 *twiddles, u128 *as0, u128 *as1);
 
 int bntt_ref_init(void) { return 0; }
 
-/* Y, X, tw: RN native u128 each (lo word first).  Returns ns per call (mean
- * over iters calls after one warm call). */
+/* Y, X, tw: RN native u128 each (lo word first).  Returns ns per call: the
+ * mean of the last half of `iters` timed calls (after one warm call). */
 double bntt_ref_ntt(u128 *Y, u128 *X, const u128 *m, const u128 *mu, u128 *tw, int iters)
 {
     u128 *as0 = (u128 *)malloc(sizeof(u128) * RN), *as1 = (u128 *)malloc(sizeof(u128) * RN);
+    double *t = (double *)malloc(sizeof(double) * (iters > 0 ? iters : 1));
     ntt(Y, X, *m, *mu, tw, as0, as1);
-    double t0 = now_ns();
-    for (int i = 0; i < iters; i++) ntt(Y, X, *m, *mu, tw, as0, as1);
-    double dt = iters > 0 ? (now_ns() - t0) / iters : 0.0;
+    for (int i = 0; i < iters; i++) {
+        const double t0 = now_ns();
+        ntt(Y, X, *m, *mu, tw, as0, as1);
+        t[i] = now_ns() - t0;
+    }
+    const double dt = iters > 0 ? mean_last_half(t, iters) : 0.0;
+    free(t);
     free(as0);
     free(as1);
     return dt;
@@ -100,7 +116,8 @@ static vec *valloc_(size_t n)
 
 /* Y, X, tw in the reference's hi/lo-u64 layout (2*RN words each).  Runs
  * ntt_pre once per call (untimed, as in the reference driver), times ntt()
- * only, and returns ns per ntt() (mean over iters after one warm call). */
+ * only, and returns ns per ntt(): the mean of the last half of `iters` timed
+ * calls (after one warm call). */
 double bntt_ref_ntt(uint64_t *Y, uint64_t *X, const u128 *m, const u128 *mu, uint64_t *tw, int iters)
 {
     const size_t v = RN / LANES, ops = RN / (2 * LANES), nv = (2 * (size_t)RN) / LANES;
@@ -113,18 +130,25 @@ double bntt_ref_ntt(uint64_t *Y, uint64_t *X, const u128 *m, const u128 *mu, uin
 #if defined(REF_NTT_AVX512)
     vec ph, pl;
     ntt_pre(X, tw, *m, *mu, tempx, twdh, twdl, &mh, &ml, &muh, &mul, &ph, &pl);
+    double *t = (double *)malloc(sizeof(double) * (iters > 0 ? iters : 1));
     ntt(tempx, twdh, twdl, mh, ml, muh, mul, ph, pl, xh, xl, ash, asl, th, tl, as);
-    double t0 = now_ns();
-    for (int i = 0; i < iters; i++)
+    for (int i = 0; i < iters; i++) {
+        const double t0 = now_ns();
         ntt(tempx, twdh, twdl, mh, ml, muh, mul, ph, pl, xh, xl, ash, asl, th, tl, as);
+        t[i] = now_ns() - t0;
+    }
 #else
+    double *t = (double *)malloc(sizeof(double) * (iters > 0 ? iters : 1));
     ntt_pre(X, tw, *m, *mu, tempx, twdh, twdl, &mh, &ml, &muh, &mul);
     ntt(tempx, twdh, twdl, mh, ml, muh, mul, xh, xl, ash, asl, th, tl, as);
-    double t0 = now_ns();
-    for (int i = 0; i < iters; i++)
+    for (int i = 0; i < iters; i++) {
+        const double t0 = now_ns();
         ntt(tempx, twdh, twdl, mh, ml, muh, mul, xh, xl, ash, asl, th, tl, as);
+        t[i] = now_ns() - t0;
+    }
 #endif
-    if (iters > 0) dt = (now_ns() - t0) / iters;
+    if (iters > 0) dt = mean_last_half(t, iters);
+    free(t);
     ntt_post(Y, as);
     free(twdh); free(twdl); free(tempx); free(as);
     free(xh); free(xl); free(ash); free(asl); free(th); free(tl);
@@ -168,6 +192,16 @@ double bntt_ref_blas(int op, u128 *c, u128 *a, u128 *b, long nchunks, const u128
         }
     }
     return now_ns() - t0;
+}
+
+double bntt_ref_blas_repeat(int op, u128 *c, u128 *a, u128 *b, int iters, const u128 *m, const u128 *mu,
+                            const u128 *x)
+{
+    double *t = (double *)malloc(sizeof(double) * (iters > 0 ? iters : 1));
+    for (int i = 0; i < iters; i++) t[i] = bntt_ref_blas(op, c, a, b, 1, m, mu, x);
+    double r = iters > 0 ? mean_last_half(t, iters) : 0.0;
+    free(t);
+    return r;
 }
 #endif
 
@@ -216,5 +250,15 @@ double bntt_ref_blas(int op, uint64_t *c, uint64_t *a, uint64_t *b, long nchunks
         }
     }
     return now_ns() - t0;
+}
+
+double bntt_ref_blas_repeat(int op, uint64_t *c, uint64_t *a, uint64_t *b, int iters, const u128 *m,
+                            const u128 *mu, const u128 *x)
+{
+    double *t = (double *)malloc(sizeof(double) * (iters > 0 ? iters : 1));
+    for (int i = 0; i < iters; i++) t[i] = bntt_ref_blas(op, c, a, b, 1, m, mu, x);
+    double r = iters > 0 ? mean_last_half(t, iters) : 0.0;
+    free(t);
+    return r;
 }
 #endif

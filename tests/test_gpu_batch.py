@@ -267,12 +267,37 @@ def test_uneven_20000_full_check(gpu, orc):
 
     wl = uneven_pairs(20000, QN, seed=2024)
     dx, dy, dc, tabs, tasks, muls = build(gpu, wl)
-    plan = BatchPlan(tasks, muls, QN, MUN, tune=load_tune_table(), graph=True)
+    # as the benchmark runs it: tuned, graph replay, tables converted at creation
+    plan = BatchPlan(tasks, muls, QN, MUN, tune=load_tune_table(), graph=True, fixed_tables=True)
     stream = gpu.cuda.Stream()
     with gpu.cuda.stream(stream):
         plan.execute()
     stream.synchronize()
     check_all(orc, wl, from_dev(dy), from_dev(dc), threads=16)
+    plan.close()
+
+
+@pytest.mark.parametrize("L", [10, 4, 5, 6, 7, 8, 9])
+def test_uniform_bench_shapes(gpu, orc, L):
+    """The uniform batches the benchmark times, at full size and with its
+    seeds: 8192 x N=1024 and 2^23 coefficients of each N = 16..512, one shared
+    table, as a tuned plan with fixed tables replayed from a graph (two
+    executions); every output against the oracle."""
+    from benchntt_amd.batch import BatchPlan, load_tune_table
+
+    total, n = 1 << 23, 1 << L
+    cnt = total // n
+    x, tw = uniform(total, QN, 5), uniform(n, QN, 40 + L)
+    dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+    dy = gpu.empty_like(dx)
+    tasks = [(dx[i * n:(i + 1) * n], dy[i * n:(i + 1) * n], dtw) for i in range(cnt)]
+    stream = gpu.cuda.Stream()
+    with gpu.cuda.stream(stream):
+        plan = BatchPlan(tasks, [], QN, MUN, tune=load_tune_table(), graph=True, fixed_tables=True)
+        plan.execute()
+        plan.execute()
+    stream.synchronize()
+    assert np.array_equal(from_dev(dy), orc.ntt(x, tw, QN, MUN))
     plan.close()
 
 

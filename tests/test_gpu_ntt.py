@@ -48,6 +48,16 @@ def test_random_batches(gpu, orc, L):
     assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
 
 
+@pytest.mark.parametrize("L", [10, 11, 12])
+@pytest.mark.parametrize("count", [1, 4, 5])
+def test_few_transforms(gpu, orc, L, count):
+    """One to five transforms of 2^10..2^12 points in one launch, one
+    workgroup per transform, against the oracle."""
+    n = 1 << L
+    x, tw = uniform(count * n, QN, 70 + L + count), uniform(n, QN, 80 + L)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
 @pytest.mark.parametrize("L", [17, 18, 20, 22, 24])
 def test_large_plain(gpu, orc, L):
     """Up to the largest size the entry points accept (2^24 points: three

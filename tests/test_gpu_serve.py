@@ -175,6 +175,58 @@ def test_alone_equals_mixed(gpu, orc, name):
     srv.close()
 
 
+def test_alone_equals_mixed_large_sizes(gpu, orc):
+    """The serving sizes of the benchmark, 2^10 .. 2^16 (two-pass transforms
+    from 2^12 on, and 2^10, 2^11 in one or two passes by batch size): each
+    request of a Poisson trace equals the same request alone, and a sample
+    covering every size matches the oracle's exact NTT pipeline."""
+    q = C.Q_116
+    srv = server(q, hbm_budget=1 << 30, max_inflight=4, max_batch=16, min_log2n=10, max_log2n=16)
+    logs, hosts, dev, arrival = _trace(gpu, 56, q, 21, 10, 16, 30_000)
+    mixed = [gpu.empty_like(a) for a, _ in dev]
+    done, st = srv.run_trace([(a, b, c) for (a, b), c in zip(dev, mixed)], arrival)
+    assert st["requests"] == 56 and st["batches"] >= 2
+    for i, ((a, b), c) in enumerate(zip(dev, mixed)):
+        alone = gpu.empty_like(a)
+        srv.multiply([(a, b, alone)])
+        assert gpu.equal(alone, c), f"request {i} (N=2^{int(logs[i])}) differs alone vs mixed"
+    for L in range(10, 17):
+        i = int(np.flatnonzero(logs == L)[0])
+        a, b = hosts[i]
+        assert np.array_equal(from_dev(mixed[i]), orc.polymul_ntt(a, b, q)), f"request {i} (N=2^{L})"
+    srv.close()
+
+
+def test_bench_serving_trace_sample(gpu, orc):
+    """The benchmark's serving trace as the benchmark builds it (16,384
+    requests, N uniform on 2^10 .. 2^16 from seed 55, operands from the
+    device generator with seeds 551 and 552, 2 GiB budget, 4 streams, batches
+    of 64, every request arrived at t = 0): a seeded sample of 14 requests,
+    two of each size, against the oracle's exact NTT pipeline."""
+    from benchntt_amd import fixtures as fx
+    from benchntt_amd.polymul import PolymulServer
+
+    q = C.Q_116
+    n_req = 16384
+    logs = np.random.default_rng(55).integers(10, 17, size=n_req)
+    sizes = [1 << int(l) for l in logs]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    A, B = fx.uniform(int(offs[-1]), q, 551), fx.uniform(int(offs[-1]), q, 552)
+    Cc = gpu.empty_like(A)
+    reqs = [(A[offs[i]:offs[i + 1]], B[offs[i]:offs[i + 1]], Cc[offs[i]:offs[i + 1]]) for i in range(n_req)]
+    srv = PolymulServer(q, hbm_budget=2 << 30, max_inflight=4, max_batch=64, min_log2n=10, max_log2n=16)
+    done, st = srv.run_trace(reqs, np.zeros(n_req, dtype=np.uint64))
+    assert st["requests"] == n_req and st["dropped"] == 0
+    gpu.cuda.synchronize()
+    rng = np.random.default_rng(1234)
+    for L in range(10, 17):
+        for i in rng.choice(np.flatnonzero(logs == L), size=2, replace=False):
+            a, b, c = (from_dev(t) for t in reqs[int(i)])
+            assert np.array_equal(c, orc.polymul_ntt(a, b, q)), f"request {int(i)} (N=2^{L})"
+    srv.close()
+    del A, B, Cc, reqs
+
+
 def test_tight_budget_admission(gpu):
     """A budget of 2.5 requests of 2^12 points: arrived requests wait for
     budget, never more than two are on the device, and results are unchanged."""

@@ -196,13 +196,36 @@ def test_oracle_matches_reference_ntt(orc, L):
 
 
 @pytest.mark.parametrize("isa", ["avx2", "avx512"])
-def test_oracle_matches_reference_avx_ntt(orc, isa):
+@pytest.mark.parametrize("L", [4, 5, 10, 11])
+def test_oracle_matches_reference_avx_ntt(orc, isa, L):
+    """The reference's AVX2 / AVX-512 ntt() at even and odd sizes (the
+    benchmark uses them at every size of its sweeps)."""
     if isa == "avx512" and ref.best_isa() != "avx512":
         pytest.skip("host has no AVX-512")
-    _need(f"ref_ntt_{isa}_L10.so")
-    x, tw = uniform(1024, QN, 1), uniform(1024, QN, 2)
+    _need(f"ref_ntt_{isa}_L{L}.so")
+    x, tw = uniform(1 << L, QN, L), uniform(1 << L, QN, 20 + L)
     y, _ = ref.ntt(x, tw, QN, MUN, isa)
     assert np.array_equal(y, orc.ntt(x, tw, QN, MUN))
+
+
+@pytest.mark.parametrize("isa", ["scalar", "avx2", "avx512"])
+def test_reference_amd_build_matches(orc, isa):
+    """The reference objects built with its AMD configuration (clang
+    -march=znver4) give the same outputs as the oracle, at an odd and an even
+    size, and the BLAS protocol entry runs."""
+    if ref.flavour() != "amd":
+        pytest.skip("no znver4 objects, or the host lacks znver4's ISA")
+    for L in (7, 12):
+        _need(f"ref_ntt_{isa}_amd_L{L}.so")
+        x, tw = uniform(1 << L, QN, 3 * L), uniform(1 << L, QN, 3 * L + 1)
+        y, ns = ref.ntt(x, tw, QN, MUN, isa, iters=4, fl="amd")
+        assert np.array_equal(y, orc.ntt(x, tw, QN, MUN)) and ns > 0
+    a, b = uniform(1024, QB, 5), uniform(1024, QB, 6)
+    ops = ref.BlasOperands(a, b, isa, fl="amd")
+    ref.blas("vvmul", ops, QB, MUB)
+    assert np.array_equal(ops.result(), orc.vvmul(a, b, QB, MUB))
+    assert ref.blas_ns_per_element("axpy", ops, QB, MUB, C.AXPY_SCALAR, iters=10) > 0
+    assert np.array_equal(ops.result(), orc.axpy(a, b, C.AXPY_SCALAR, QB, MUB))
 
 
 def test_oracle_matches_reference_blas(orc):
