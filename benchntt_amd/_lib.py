@@ -86,7 +86,8 @@ class PmStats(ctypes.Structure):
 
     _fields_ = [("requests", ctypes.c_uint64), ("batches", ctypes.c_uint64),
                 ("max_inflight_requests", ctypes.c_uint64), ("admission_waits", ctypes.c_uint64),
-                ("dropped", ctypes.c_uint64)]
+                ("dropped", ctypes.c_uint64), ("max_batch_ns", ctypes.c_uint64),
+                ("max_loop_gap_ns", ctypes.c_uint64)]
 
 
 # done_ns of a request that bntt_pm_run_trace dropped

@@ -20,7 +20,8 @@
 //            and twiddle addresses are fixed per item
 //   QK_MONT  exact Montgomery arithmetic (polynomial-multiply server);
 //            QK_MONT_S the same for a modulus whose 31-bit digits 1 and 2
-//            are zero (a shorter reduction)
+//            are zero (a shorter reduction); QK_MONT8 / QK_MONT_S8 the same
+//            with the per-stage-count code of QK_FAST8
 // Keeping the literal formulas out of the fast kernel keeps its registers
 // free of spills.  Kinds of item:
 //   ITEM_NTT   stages [S, S+k) of the forward network for G groups (several
@@ -107,10 +108,20 @@ struct QueueCtl {
     uint32_t pad[2];
 };
 
-enum QueueKind : int { QK_FAST = 0, QK_SLOW = 1, QK_MONT = 2, QK_MONT_S = 3, QK_FAST8 = 4 };
-__host__ __device__ constexpr bool qk_mont(int kind) { return kind == QK_MONT || kind == QK_MONT_S; }
+enum QueueKind : int { QK_FAST = 0, QK_SLOW = 1, QK_MONT = 2, QK_MONT_S = 3, QK_FAST8 = 4, QK_MONT8 = 5, QK_MONT_S8 = 6 };
+__host__ __device__ constexpr bool qk_mont(int kind)
+{
+    return kind == QK_MONT || kind == QK_MONT_S || kind == QK_MONT8 || kind == QK_MONT_S8;
+}
+// the shorter Montgomery reduction (q's 31-bit digits 1 and 2 are zero)
+__host__ __device__ constexpr bool qk_sparse(int kind) { return kind == QK_MONT_S || kind == QK_MONT_S8; }
 // fast-path reference arithmetic: items that need the literal formulas are punted
 __host__ __device__ constexpr bool qk_fast(int kind) { return kind == QK_FAST || kind == QK_FAST8; }
+// launches of 2048-value radix-8 items run code specialised per stage count
+__host__ __device__ constexpr bool qk_item8(int kind)
+{
+    return kind == QK_FAST8 || kind == QK_MONT8 || kind == QK_MONT_S8;
+}
 
 // Items a fast launch left to the literal formulas: count is cleared before
 // each execution, idx holds up to one entry per item of the launch's list.
@@ -140,8 +151,8 @@ __device__ __forceinline__ void run_network_elog(U128 *lds, const TwView &tv, in
                                                  uint32_t pmask, const Mod &md)
 {
     if constexpr (qk_mont(KIND)) {
-        constexpr int M = KIND == QK_MONT_S ? TW_MONT_S : TW_MONT;
-        constexpr int ML = KIND == QK_MONT_S ? TW_MONT_S_LDS : TW_MONT_LDS;
+        constexpr int M = qk_sparse(KIND) ? TW_MONT_S : TW_MONT;
+        constexpr int ML = qk_sparse(KIND) ? TW_MONT_S_LDS : TW_MONT_LDS;
         if (klds > 0)
             run_group_network<ELOG, ML, true, M>(lds, tv, k, S, mbase, md, klds, pmask);
         else
@@ -287,7 +298,7 @@ __device__ __forceinline__ bool load_tile(U128 *lds, const U128 *const *s_x, con
                                           const WorkItem &it, const DevDesc &d0, const U128 *src, const U128 *srcB,
                                           const Mod &md)
 {
-    constexpr bool MONT = qk_mont(KIND), SPARSE = KIND == QK_MONT_S;
+    constexpr bool MONT = qk_mont(KIND), SPARSE = qk_sparse(KIND);
     const int L = it.L, k = it.k, Glog = it.Glog, mlog = L - k, nvalid = (int)it.nvalid;
     const int G = 1 << Glog;
     // groups of one transform in the tile (a paired item holds two)
@@ -428,7 +439,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
                             uint64_t *s_twkey, const WorkItem &it, const DevDesc *__restrict__ descs, TwArea area,
                             const Mod &md)
 {
-    constexpr bool MONT = qk_mont(KIND), SPARSE = KIND == QK_MONT_S;
+    constexpr bool MONT = qk_mont(KIND), SPARSE = qk_sparse(KIND);
     const int L = it.L, S = it.S, k = it.k, Glog = it.Glog, elog = it.elog;
     const int E = 1 << elog;
     const int mlog = L - k;
@@ -604,7 +615,7 @@ __device__ bool do_ntt_item(U128 *lds, const U128 **s_x, U128 **s_y, const U128 
 // costs little beyond its butterflies.
 
 // One register sub-pass: local stages [r0, r0 + KP) of the tile's groups.
-template <int K, int KP>
+template <int K, int KP, int KIND>
 __device__ __forceinline__ void subpass8(U128 *lds, const R31 *__restrict__ twr, int S, int r0, uint32_t mbase,
                                          uint32_t pmask, int tid, const Mod &md)
 {
@@ -638,7 +649,10 @@ __device__ __forceinline__ void subpass8(U128 *lds, const R31 *__restrict__ twr,
         const int sg = tid + q * kQueueThreads;
         const int g = sg >> SL;
         const int Mp = sg & ((1 << SL) - 1);
-        reg_network_fast<KP>(v[q], t[q], md);
+        if constexpr (qk_mont(KIND))
+            reg_network_mont<KP, qk_sparse(KIND)>(v[q], t[q], md);
+        else
+            reg_network_fast<KP>(v[q], t[q], md);
         // outputs of the sub-group: R consecutive values of one aligned block
         // of R (inside one aligned block of 8), so consecutive slots
         const int base = lidx((g << K) + (Mp << KP), K);
@@ -648,10 +662,11 @@ __device__ __forceinline__ void subpass8(U128 *lds, const R31 *__restrict__ twr,
     __syncthreads();
 }
 
-template <int K>
-__device__ __forceinline__ bool do_ntt_item8(U128 *lds, const U128 **s_x, U128 **s_y, U128 **s_c, const WorkItem &it,
-                             const DevDesc *__restrict__ descs, const Mod &md)
+template <int K, int KIND>
+__device__ __forceinline__ bool do_ntt_item8(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2, U128 **s_c,
+                                             const WorkItem &it, const DevDesc *__restrict__ descs, const Mod &md)
 {
+    constexpr bool MONT = qk_mont(KIND), SPARSE = qk_sparse(KIND);
     constexpr int T = kQueueThreads;
     constexpr int Glog = 11 - K, G = 1 << Glog;
     constexpr int E = 8;
@@ -664,21 +679,28 @@ __device__ __forceinline__ bool do_ntt_item8(U128 *lds, const U128 **s_x, U128 *
     const int L = it.L, S = it.S, mlog = L - K;
     const int nvalid = (int)it.nvalid;
     const DevDesc d0 = descs[it.desc];
-    const bool pair = (it.flags & IT_PAIR) != 0;
+    const bool pair = !MONT && (it.flags & IT_PAIR);
     const int plog = Glog - (pair ? 1 : 0);
     const uint32_t pmask = (1u << plog) - 1u;
     const int H = 1 << plog;
     const bool prod = pair && (it.flags & IT_PROD) && it.pass + 1 == it.np;
-    const bool contig = (it.flags & IT_CONTIG) != 0;
+    const bool contig = !MONT && (it.flags & IT_CONTIG);
+    // fused pointwise steps of the Montgomery path (a plain gather is a
+    // product with B = 1)
+    const bool gather = MONT && (it.flags & (IT_GATHER_MUL | IT_GATHER)) && it.pass == 0;
+    const bool gmul = (it.flags & IT_GATHER_MUL) != 0;
+    const bool scatter = MONT && (it.flags & IT_SCATTER_SCALE) && it.pass + 1 == it.np;
+    const bool canon = MONT && (it.flags & IT_CANON) && it.pass + 1 == it.np;
+    const U128 one{{1u, 0u, 0u, 0u}};
     const U128 *src, *srcB = nullptr;
     U128 *dst, *dstB = nullptr;
     pass_buffers(it, d0, src, dst);
     if (pair && mlog > 0) pass_buffers(it, descs[it.desc + 1], srcB, dstB);
-    if (mlog == 0 && !contig) item_buffers<false>(s_x, s_y, nullptr, s_c, it, descs);
+    if (mlog == 0 && !contig) item_buffers<MONT>(s_x, s_y, s_x2, s_c, it, descs);
 
     // HBM -> LDS, every load of a thread issued before the first is used (an
     // element outside the kept groups reads a kept one and is dropped)
-    U128 a[E];
+    U128 a[E], b[E];
     bool big = false;
     if (contig) {
 #pragma unroll
@@ -695,20 +717,50 @@ __device__ __forceinline__ bool do_ntt_item8(U128 *lds, const U128 **s_x, U128 *
             }
         }
     } else if (mlog == 0) {
+        // whole transforms, one per group, each with its own buffers
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int e = tid + j * T;
             const int g = e >> K, ti = e & ((1 << K) - 1);
             const int gv = (int)((uint32_t)g & pmask) < nvalid ? g : 0;
             a[j] = ld16(s_x[gv] + ti);
+            if (MONT && gather && gmul) b[j] = ld16(s_x2[gv] + ti);
         }
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int e = tid + j * T;
-            if ((int)((uint32_t)(e >> K) & pmask) < nvalid) {
-                big |= near_q(a[j], md.m);
-                lds[lidx(e, K)] = a[j];
+            const int g = e >> K, ti = e & ((1 << K) - 1);
+            if ((int)((uint32_t)g & pmask) < nvalid) {
+                if (MONT && gather) {
+                    // network input brv(ti) is A[ti] * B[ti]: contiguous loads,
+                    // bit-reversed placement
+                    lds[lidx((g << K) + (int)bitrev((uint32_t)ti, K), K)] =
+                        gather_product<SPARSE>(a[j], gmul ? b[j] : one, md.mq);
+                } else {
+                    big |= near_q(a[j], md.m);
+                    lds[lidx(e, K)] = a[j];
+                }
             }
+        }
+    } else if (MONT && gather) {
+        // group g's inputs, bit-reversed, are the contiguous block
+        // brv(M) << K of A and B (in the order brv(ti))
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = tid + j * T;
+            const int g = e >> K, jj = e & ((1 << K) - 1);
+            const uint32_t gv = (uint32_t)(g < nvalid ? g : 0);
+            const uint32_t blk = bitrev((uint32_t)it.first + gv, (uint32_t)mlog) << K;
+            a[j] = ld16(d0.x + blk + jj);
+            if (gmul) b[j] = ld16(d0.x2 + blk + jj);
+        }
+#pragma unroll
+        for (int j = 0; j < E; j++) {
+            const int e = tid + j * T;
+            const int g = e >> K, jj = e & ((1 << K) - 1);
+            if (g < nvalid)
+                lds[lidx((g << K) + (int)bitrev((uint32_t)jj, K), K)] =
+                    gather_product<SPARSE>(a[j], gmul ? b[j] : one, md.mq);
         }
     } else if (second_of_two(it)) {
         // contiguous groups (the first pass stored them so)
@@ -746,15 +798,22 @@ __device__ __forceinline__ bool do_ntt_item8(U128 *lds, const U128 **s_x, U128 *
             }
         }
     }
-    // a large twiddle, or a value above q: the literal formulas (the vote is
-    // also the barrier after the tile writes)
-    const bool twb = *(const __attribute__((address_space(1))) uint32_t *)d0.twflag != 0;
-    if (__syncthreads_or(big || twb) && (twb || tile_above_q_kept(lds, K, Glog, pmask, nvalid, md.m))) return false;
+    if constexpr (!MONT) {
+        // a large twiddle, or a value above q: the literal formulas (the vote
+        // is also the barrier after the tile writes)
+        const bool twb = *(const __attribute__((address_space(1))) uint32_t *)d0.twflag != 0;
+        if (__syncthreads_or(big || twb) && (twb || tile_above_q_kept(lds, K, Glog, pmask, nvalid, md.m)))
+            return false;
+    } else {
+        // Montgomery: inputs are canonical, no literal path
+        (void)big;
+        __syncthreads();
+    }
 
     const uint32_t mbase = (uint32_t)it.first;
 #pragma unroll 1
-    for (int r0 = 0; r0 + 3 <= K; r0 += 3) subpass8<K, 3>(lds, d0.twr, S, r0, mbase, pmask, tid, md);
-    if constexpr (K % 3 != 0) subpass8<K, K % 3>(lds, d0.twr, S, K - K % 3, mbase, pmask, tid, md);
+    for (int r0 = 0; r0 + 3 <= K; r0 += 3) subpass8<K, 3, KIND>(lds, d0.twr, S, r0, mbase, pmask, tid, md);
+    if constexpr (K % 3 != 0) subpass8<K, K % 3, KIND>(lds, d0.twr, S, K - K % 3, mbase, pmask, tid, md);
 
     // LDS -> HBM
     if (pair) {
@@ -799,47 +858,71 @@ __device__ __forceinline__ bool do_ntt_item8(U128 *lds, const U128 **s_x, U128 *
         return true;
     }
     if (contig || mlog > 0) {
-        // the kept groups' outputs are one contiguous range
-        U128 *yb = contig ? d0.y : dst + ((uint32_t)it.first << K);
+        // the kept groups' outputs are one contiguous range (in network
+        // order; a Montgomery scatter stores output idx at brv(idx))
+        const uint32_t first = (uint32_t)it.first << K;
+        U128 *yb = contig ? d0.y : dst + first;
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int e = tid + j * T;
-            if ((e >> K) < nvalid) st16(yb + e, lds[lidx(e, K)]);
+            if ((e >> K) < nvalid) {
+                const U128 v = lds[lidx(e, K)];
+                if (MONT && scatter)
+                    st16(dst + bitrev(first + (uint32_t)e, (uint32_t)L),
+                         montmul_r31<SPARSE>(v, ldr31(d0.scale + first + (uint32_t)e), md.mq));
+                else if (MONT && canon)
+                    st16(yb + e, canonical(v, md.mq));
+                else
+                    st16(yb + e, v);
+            }
         }
     } else {
 #pragma unroll
         for (int j = 0; j < E; j++) {
             const int e = tid + j * T;
             const int g = e >> K;
-            if (g < nvalid) st16(s_y[g] + (e & ((1 << K) - 1)), lds[lidx(e, K)]);
+            if (g < nvalid) {
+                const uint32_t ti = (uint32_t)(e & ((1 << K) - 1));
+                if (MONT && scatter) {
+                    // position ti of c takes network output brv(ti): contiguous
+                    // stores, factor from the table permuted to positions
+                    const U128 v = lds[lidx((g << K) + (int)bitrev(ti, K), K)];
+                    st16(s_y[g] + ti, montmul_r31<SPARSE>(v, ldr31(d0.scale + (1u << K) + ti), md.mq));
+                } else {
+                    const U128 v = lds[lidx(e, K)];
+                    st16(s_y[g] + ti, (MONT && canon) ? canonical(v, md.mq) : v);
+                }
+            }
         }
     }
     return true;
 }
 
-// A QK_FAST8 transform item: the code for its stage count.
-__device__ __forceinline__ bool do_ntt_item8_k(U128 *lds, const U128 **s_x, U128 **s_y, U128 **s_c,
-                                               const WorkItem &it, const DevDesc *__restrict__ descs, const Mod &md)
+// A transform item of a launch of kind qk_item8: the code for its stage count.
+template <int KIND>
+__device__ __forceinline__ bool do_ntt_item8_k(U128 *lds, const U128 **s_x, U128 **s_y, const U128 **s_x2,
+                                               U128 **s_c, const WorkItem &it, const DevDesc *__restrict__ descs,
+                                               const Mod &md)
 {
     switch (it.k) {
     case 3:
-        return do_ntt_item8<3>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<3, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 4:
-        return do_ntt_item8<4>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<4, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 5:
-        return do_ntt_item8<5>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<5, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 6:
-        return do_ntt_item8<6>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<6, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 7:
-        return do_ntt_item8<7>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<7, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 8:
-        return do_ntt_item8<8>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<8, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 9:
-        return do_ntt_item8<9>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<9, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     case 10:
-        return do_ntt_item8<10>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<10, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     default:
-        return do_ntt_item8<11>(lds, s_x, s_y, s_c, it, descs, md);
+        return do_ntt_item8<11, KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
     }
 }
 
@@ -931,8 +1014,8 @@ __global__ __launch_bounds__(kQueueThreads, 3) void queue_kernel(const WorkItem 
         const WorkItem it = items[(KIND == QK_SLOW && punt.idx) ? punt.idx[t] : t];
         bool done = true;
         if (it.kind == ITEM_NTT) {
-            if constexpr (KIND == QK_FAST8)
-                done = do_ntt_item8_k(lds, s_x, s_y, s_c, it, descs, md);
+            if constexpr (qk_item8(KIND))
+                done = do_ntt_item8_k<KIND>(lds, s_x, s_y, s_x2, s_c, it, descs, md);
             else
                 done = do_ntt_item<KIND>(lds, s_x, s_y, s_x2, s_c, s_twkey, it, descs, area, md);
         } else if constexpr (!MONT) {

@@ -161,6 +161,7 @@ struct Slot {
     char *stage_h = nullptr;  // pinned staging of the same layout
     size_t cap_items = 0, cap_descs = 0;
     bool busy = false;
+    uint64_t launched_ns = 0;                          // trace clock at the batch's launch
     std::vector<size_t> reqs;                          // request indices of the batch
     std::vector<std::pair<size_t, size_t>> allocs;     // arena blocks
     // work lists being built for the slot's next batch: kept across batches
@@ -190,14 +191,20 @@ struct bntt_pm_server {
 
 namespace {
 
-// The serving kernel for the server's modulus.
+// The serving kernel for the server's modulus; k8: every item of the level
+// is a 2048-value radix-8 tile, run by the code specialised per stage count.
 template <typename... A>
-void launch_serving(const bntt_pm_server *s, dim3 grid, size_t lds, hipStream_t st, A... args)
+void launch_serving(const bntt_pm_server *s, bool k8, dim3 grid, size_t lds, hipStream_t st, A... args)
 {
-    if (s->sparse)
-        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT_S>, grid, dim3(bntt::kQueueThreads), lds, st, args...);
+    const dim3 block(bntt::kQueueThreads);
+    if (s->sparse && k8)
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT_S8>, grid, block, lds, st, args...);
+    else if (s->sparse)
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT_S>, grid, block, lds, st, args...);
+    else if (k8)
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT8>, grid, block, lds, st, args...);
     else
-        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT>, grid, dim3(bntt::kQueueThreads), lds, st, args...);
+        hipLaunchKernelGGL(bntt::queue_kernel<bntt::QK_MONT>, grid, block, lds, st, args...);
 }
 
 }  // namespace
@@ -280,7 +287,10 @@ int build_tables(bntt_pm_server *s, int L)
 int make_slot(Slot &sl, size_t cap_items, size_t cap_descs)
 {
     if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess)
+        // completion only: the results are device memory, which the kernels'
+        // own end-of-kernel release makes visible to every stream, so the
+        // event needs no system-scope fence (a cache write-back per batch)
+        hipEventCreateWithFlags(&sl.done, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess)
         return bntt_set_error(BNTT_EHIP, "creating a serving stream failed");
     sl.cap_items = cap_items;
     sl.cap_descs = cap_descs;
@@ -440,14 +450,16 @@ int launch_levels(bntt_pm_server *s, Slot &sl, const std::vector<std::vector<Wor
         if (lv[l].empty()) continue;
         // the largest data tile and staged-table need of the level's items
         size_t data_slots = 0, need = 0;
+        bool all8 = true;
         for (const WorkItem &w : lv[l]) {
             data_slots = std::max(data_slots, bntt::lds_data_slots(w.k, w.Glog));
             need = std::max(need, bntt::tw_slots(w.k, bntt::item_tw_wlog(w.L, w.k, w.S, w.Glog)));
+            all8 = all8 && w.elog == 3 && w.k >= 3;
         }
         bntt::TwArea area;
         const size_t lds = bntt::plan_tw_area(data_slots, &need, 1, s->tw_budget, &area);
         const int grid = (int)std::min<size_t>(lv[l].size(), (size_t)(lds > bntt::kQueueLds3 ? s->grid_cap2 : s->grid_cap));
-        launch_serving(s, dim3(grid), lds, sl.stream, reinterpret_cast<const WorkItem *>(sl.batch_d) + off[l],
+        launch_serving(s, all8 && s->tw_budget == 0, dim3(grid), lds, sl.stream, reinterpret_cast<const WorkItem *>(sl.batch_d) + off[l],
                        (uint32_t)lv[l].size(), reinterpret_cast<const DevDesc *>(sl.batch_d + desc_off),
                        static_cast<const bntt::DevMul *>(nullptr), sl.ctl, area, s->md,
                        bntt::PuntList{nullptr, nullptr});
@@ -544,18 +556,26 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
     s->sparse = s->md.mq.q.d[1] == 0 && s->md.mq.q.d[2] == 0;
     const void *kfn = s->sparse ? reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT_S>)
                                 : reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT>);
+    const void *kfn8 = s->sparse ? reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT_S8>)
+                                 : reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT8>);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, s->dev) != hipSuccess ||
-        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
+        hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess ||
+        hipFuncSetAttribute(kfn8, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bntt::kQueueMaxLds) != hipSuccess) {
         destroy_server(s);
         return bntt_set_error(BNTT_EHIP, "querying the device for the serving kernel failed");
     }
-    int per_cu = 0, per_cu2 = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, bntt::kQueueThreads, bntt::kQueueLds3) != hipSuccess)
-        per_cu = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu2, kfn, bntt::kQueueThreads, bntt::kQueueMaxLds) !=
-        hipSuccess)
-        per_cu2 = 1;
+    // resident workgroups: the smaller of the two kernels' figures
+    int per_cu = 1 << 20, per_cu2 = 1 << 20;
+    for (const void *f : {kfn, kfn8}) {
+        int a = 0, b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, f, bntt::kQueueThreads, bntt::kQueueLds3) != hipSuccess)
+            a = 1;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, f, bntt::kQueueThreads, bntt::kQueueMaxLds) != hipSuccess)
+            b = 1;
+        per_cu = std::min(per_cu, a);
+        per_cu2 = std::min(per_cu2, b);
+    }
     s->grid_cap = prop.multiProcessorCount * std::max(per_cu, 1);
     s->grid_cap2 = prop.multiProcessorCount * std::max(per_cu2, 1);
     if (hipDeviceSynchronize() != hipSuccess) {
@@ -700,9 +720,12 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
         }
         return rc;
     };
+    uint64_t t_prev = 0;
     while (completed < n) {
         bool progress = false;
         const uint64_t t = now_ns();
+        st.max_loop_gap_ns = std::max(st.max_loop_gap_ns, t - t_prev);
+        t_prev = t;
         while (next < n && arrival_ns[next] <= t) pending.push_back(next++);
         // retire finished batches
         for (int k = 0; k < s->nslots; k++) {
@@ -712,6 +735,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
             if (q == hipErrorNotReady) continue;
             if (q != hipSuccess) return abandon(bntt_set_hip_error(q, "serving batch failed"));
             const uint64_t tdone = now_ns();
+            st.max_batch_ns = std::max(st.max_batch_ns, tdone - sl.launched_ns);
             for (size_t r : sl.reqs) done_ns[r] = tdone;
             for (auto &a : sl.allocs) s->arena.release(a.first, a.second);
             completed += sl.reqs.size();
@@ -752,6 +776,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
                                         static_cast<U128 *>(reqs[r].c), reqs[r].log2n, off});
             }
             if (views.empty()) break;  // budget full: wait for a batch to retire
+            sl.launched_ns = now_ns();
             const int rc = launch_batch(s, sl, views);
             if (rc) return abandon(rc);
             sl.busy = true;

@@ -91,7 +91,8 @@ class PolymulServer:
                   done.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(st))
         return done, {"requests": st.requests, "batches": st.batches,
                       "max_inflight_requests": st.max_inflight_requests, "admission_waits": st.admission_waits,
-                      "dropped": st.dropped}
+                      "dropped": st.dropped, "max_batch_ns": st.max_batch_ns,
+                      "max_loop_gap_ns": st.max_loop_gap_ns}
 
     def close(self) -> None:
         if self._h:

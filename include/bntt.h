@@ -244,6 +244,8 @@ typedef struct bntt_pm_stats {
     uint64_t max_inflight_requests; /* most requests on the device at once */
     uint64_t admission_waits;       /* requests that arrived and waited for budget */
     uint64_t dropped;               /* requests not admitted within admit_timeout_ns */
+    uint64_t max_batch_ns;          /* longest batch, launch to completion seen by the host */
+    uint64_t max_loop_gap_ns;       /* longest interval between two scheduler passes (a host stall) */
 } bntt_pm_stats;
 
 typedef struct bntt_pm_server bntt_pm_server;

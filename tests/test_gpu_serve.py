@@ -200,7 +200,7 @@ def test_alone_equals_mixed_large_sizes(gpu, orc):
 def test_bench_serving_trace_sample(gpu, orc):
     """The benchmark's serving trace as the benchmark builds it (16,384
     requests, N uniform on 2^10 .. 2^16 from seed 55, operands from the
-    device generator with seeds 551 and 552, 2 GiB budget, 4 streams, batches
+    device generator with seeds 551 and 552, 2 GiB budget, 3 streams, batches
     of 64, every request arrived at t = 0): a seeded sample of 14 requests,
     two of each size, against the oracle's exact NTT pipeline."""
     from benchntt_amd import fixtures as fx
@@ -214,9 +214,10 @@ def test_bench_serving_trace_sample(gpu, orc):
     A, B = fx.uniform(int(offs[-1]), q, 551), fx.uniform(int(offs[-1]), q, 552)
     Cc = gpu.empty_like(A)
     reqs = [(A[offs[i]:offs[i + 1]], B[offs[i]:offs[i + 1]], Cc[offs[i]:offs[i + 1]]) for i in range(n_req)]
-    srv = PolymulServer(q, hbm_budget=2 << 30, max_inflight=4, max_batch=64, min_log2n=10, max_log2n=16)
+    srv = PolymulServer(q, hbm_budget=2 << 30, max_inflight=3, max_batch=64, min_log2n=10, max_log2n=16)
     done, st = srv.run_trace(reqs, np.zeros(n_req, dtype=np.uint64))
     assert st["requests"] == n_req and st["dropped"] == 0
+    assert 0 < st["max_batch_ns"] <= int(done.max()) and st["max_loop_gap_ns"] > 0
     gpu.cuda.synchronize()
     rng = np.random.default_rng(1234)
     for L in range(10, 17):
