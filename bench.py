@@ -342,10 +342,10 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
         tasks = [(xb[i * n:(i + 1) * n], yb[i * n:(i + 1) * n], twl) for i in range(cnt)]
         with torch.cuda.stream(s2):
             plan = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True, fixed_tables=True)
-            msp = time_kernel(torch, plan.execute, 20, 3, s2)
+            msp = time_kernel(torch, plan.execute, 200, 200, s2)
             plan.close()
             if L == 10:
-                msd = time_kernel(torch, lambda: bn.ntt(yb, xb, qn, mun, twl), 20, 3, s2)
+                msd = time_kernel(torch, lambda: bn.ntt(yb, xb, qn, mun, twl), 200, 200, s2)
                 tl = TaskList(tasks)
                 mse = time_kernel(torch, lambda: launch_each(tl, qn, mun), 2, 1, s2) if per_launch else float("nan")
         key = f"ntt_uniform_{cnt}x{n}"
@@ -439,8 +439,8 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     with torch.cuda.stream(s2):
         full = BatchPlan(tasks, muls, qn, mun, tune=tune, graph=True, fixed_tables=True)
         only = BatchPlan(tasks, [], qn, mun, tune=tune, graph=True, fixed_tables=True)
-        ms_full = time_kernel(torch, full.execute, 5, 2, s2)
-        ms_only = time_kernel(torch, only.execute, 5, 2, s2)
+        ms_full = time_kernel(torch, full.execute, 20, 10, s2)
+        ms_only = time_kernel(torch, only.execute, 20, 10, s2)
         tmp = torch.empty((1 << 16, 2), dtype=torch.int64, device=dev)
         tl = TaskList(tasks)
         ms_each = time_kernel(torch, lambda: launch_each(tl, qn, mun, tmp), 1, 1, s2) if per_launch else float("nan")
@@ -577,10 +577,11 @@ def main() -> int:
     ap.add_argument("--skip-serving", action="store_true", help="leave out the serving trace")
     ap.add_argument("--skip-per-launch", action="store_true",
                     help="skip the one-launch-per-transform comparisons (keeps profiler output small)")
-    ap.add_argument("--only", choices=["all", "axpy", "copy", "ntt-uniform", "uneven", "serving", "sweep", "blas1024"],
+    ap.add_argument("--only", choices=["all", "axpy", "vvadd", "vvsub", "vvmul", "copy", "ntt-uniform", "uneven",
+                                       "serving", "sweep", "blas1024"],
                     default="all",
                     help="time one workload only and print no JSON line (for profiling runs): "
-                         "axpy 2^24; copy (a 256 MiB device copy, the traffic calibration kernel); "
+                         "axpy, vvadd, vvsub or vvmul at 2^24; copy (a 256 MiB device copy, the traffic calibration kernel); "
                          "ntt-uniform (8192 x N=1024); uneven (20,000-transform batch); "
                          "serving (polynomial-multiply trace); sweep (plain NTT N = 2^8..2^20 beside the "
                          "reference's CPU ntt()); blas1024 (the reference's n = 1024 BLAS protocol)")
@@ -638,6 +639,12 @@ def main() -> int:
     dc = torch.empty_like(da)
     torch.cuda.synchronize()
 
+    vv = {"vvadd": lambda: bn.vector_add(dc, da, db, q), "vvsub": lambda: bn.vector_sub(dc, da, db, q),
+          "vvmul": lambda: bn.vector_mul(dc, da, db, q, mu)}
+    if args.only in vv:
+        ms = time_kernel(torch, vv[args.only], args.steps, args.warmup, stream)
+        log(f"{args.only} 2^24: {ms * 1e3:.1f} us/call, {n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9:.1f} GB/s")
+        return 0
     ms_axpy = time_kernel(torch, lambda: bn.axpy(dc, da, db, s, q, mu), args.steps, args.warmup, stream)
     gbs = n * BYTES_PER_ELEM / (ms_axpy * 1e-3) / 1e9
     log(f"axpy 2^24: {ms_axpy * 1e3:.1f} us/call, {gbs:.1f} GB/s")
@@ -646,11 +653,10 @@ def main() -> int:
 
     extras = {}
     if args.only == "all":
-        for name, fn in (("vvadd", lambda: bn.vector_add(dc, da, db, q)),
-                         ("vvsub", lambda: bn.vector_sub(dc, da, db, q)),
-                         ("vvmul", lambda: bn.vector_mul(dc, da, db, q, mu))):
+        for name, fn in vv.items():
             ms = time_kernel(torch, fn, args.steps, args.warmup, stream)
             extras[f"{name}_GBps"] = round(n * BYTES_PER_ELEM / (ms * 1e-3) / 1e9, 1)
+            extras[f"{name}_hbm_bytes_per_launch"] = load_traffic(f"{name}_2_24")
             log(f"{name} 2^24: {ms * 1e3:.1f} us/call, {extras[f'{name}_GBps']} GB/s")
         # what the device sustains for a plain copy (the runtime's own copy
         # kernel, 256 MiB read + 256 MiB written): the practical HBM ceiling
