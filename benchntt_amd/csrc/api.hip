@@ -7,6 +7,7 @@
 // CPU path.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -211,6 +212,9 @@ struct Staging {
     void *hbuf = nullptr;  // pinned host memory
     void *hdev = nullptr;  // its address for kernels
     size_t hcap = 0;
+    uint32_t *flag_h = nullptr;  // pinned completion word of one-workgroup launches
+    uint32_t *flag_d = nullptr;  // its address for kernels
+    uint32_t seq = 0;            // last sequence number posted to it
 };
 std::mutex g_stages_mu;
 std::map<int, std::unique_ptr<Staging>> g_stages;
@@ -251,6 +255,44 @@ int stage_reserve(Staging &stg, size_t bytes)
         if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
         stg.cap = bytes;
     }
+    return BNTT_OK;
+}
+
+// The pinned completion word of the staging (allocated at first use).
+int stage_flag(Staging &stg)
+{
+    if (stg.flag_h) return BNTT_OK;
+    void *h = nullptr, *d = nullptr;
+    hipError_t e = hipHostMalloc(&h, 64, hipHostMallocDefault);
+    if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipHostMalloc(64): %s", hipGetErrorString(e));
+    e = hipHostGetDevicePointer(&d, h, 0);
+    if (e != hipSuccess) {
+        (void)hipHostFree(h);
+        return hip_fail(e, "hipHostGetDevicePointer");
+    }
+    stg.flag_h = static_cast<uint32_t *>(h);
+    stg.flag_d = static_cast<uint32_t *>(d);
+    __atomic_store_n(stg.flag_h, stg.seq, __ATOMIC_RELEASE);
+    return BNTT_OK;
+}
+
+// Waits for a launch on the staging's stream that posts `seq` to its
+// completion word.  The host sees that store several microseconds before the
+// stream's own completion, so it spins on the word, for at most about 2 ms;
+// only if the word has not arrived by then is the stream synchronised (which
+// also reports a failed launch).
+int wait_flag(Staging &stg, uint32_t seq)
+{
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    for (uint32_t i = 1;; i++) {
+        if (__atomic_load_n(stg.flag_h, __ATOMIC_ACQUIRE) == seq) return BNTT_OK;
+        if ((i & 255u) == 0 && clk::now() - t0 > std::chrono::milliseconds(2)) break;
+    }
+    const hipError_t e = hipStreamSynchronize(stg.stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (__atomic_load_n(stg.flag_h, __ATOMIC_ACQUIRE) != seq)
+        return fail(BNTT_EHIP, "the transform completed without posting its completion word");
     return BNTT_OK;
 }
 
@@ -459,15 +501,19 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
     if (rc) return rc;
     hipStream_t s = stg.stream;
     if (tmpb == 0 && bytes <= kZeroCopyBytes) {
-        // one pass: the kernel reads x and the table once and writes y once
-        if ((rc = stage_host_reserve(stg, 3 * bytes))) return rc;
+        // one pass, one workgroup: the kernel reads x and the table once from
+        // pinned memory, writes y there once and posts its completion word
+        if ((rc = stage_host_reserve(stg, 3 * bytes)) || (rc = stage_flag(stg))) return rc;
         char *hb = static_cast<char *>(stg.hbuf), *hd = static_cast<char *>(stg.hdev);
         std::memcpy(hb, X, bytes);
         std::memcpy(hb + bytes, tw, bytes);
-        rc = bntt_ntt_fwd_dev(hd + 2 * bytes, hd, hd + bytes, log2n, 1, m, mu, nullptr, s);
-        if (rc) return rc;
-        const hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        const uint32_t seq = ++stg.seq;
+        const hipError_t e =
+            bntt::launch_ntt_uniform(reinterpret_cast<U128 *>(hd + 2 * bytes), reinterpret_cast<const U128 *>(hd),
+                                     reinterpret_cast<const U128 *>(hd + bytes), log2n, 1, make_mod(m, mu), nullptr,
+                                     s, stg.flag_d, seq);
+        if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
+        if ((rc = wait_flag(stg, seq))) return rc;
         std::memcpy(Y, hb + 2 * bytes, bytes);
         return BNTT_OK;
     }

@@ -13,7 +13,8 @@ namespace bntt {
 template <int ELOG, bool PM>
 __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict__ y, const U128 *__restrict__ x,
                                                                const U128 *__restrict__ tw, int L, int S, int k,
-                                                               int Glog, int klds, size_t total_groups, Mod md)
+                                                               int Glog, int klds, size_t total_groups, Mod md,
+                                                               uint32_t *done, uint32_t seq)
 {
     extern __shared__ U128 lds[];
     constexpr int E = 1 << ELOG;
@@ -100,11 +101,19 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
         const int e = threadIdx.x + j * T;
         if ((e >> k) < nvalid) st16(yb + e, lds[lidx(e, k)]);
     }
+    if (done) {
+        // a one-workgroup launch whose completion the host reads from pinned
+        // memory: every wave makes its stores visible system-wide, then one
+        // release store posts the sequence number
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 template <int ELOG, bool PM>
 static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int S, int k, int Glog, size_t count,
-                              const Mod &md, hipStream_t stream)
+                              const Mod &md, hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0)
 {
     const size_t total_groups = count << (L - k);
     const size_t blocks = (total_groups + ((size_t)1 << Glog) - 1) >> Glog;
@@ -117,27 +126,29 @@ static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int
     const size_t lds_bytes = pass_lds_bytes(k, S, Glog, klds);
     if (threads < 1 || threads > kMaxThreads || blocks > 0x7fffffffu || lds_bytes > kMaxLdsBytes)
         return hipErrorInvalidConfiguration;
+    if (done && blocks != 1) return hipErrorInvalidValue;
     // raise the dynamic LDS limit once per instantiation and device
     static std::atomic<uint64_t> lds_done{0};
     const hipError_t attr =
         bntt_allow_lds(reinterpret_cast<const void *>(&ntt_pass_kernel<ELOG, PM>), (int)kMaxLdsBytes, lds_done);
     if (attr != hipSuccess) return attr;
     hipLaunchKernelGGL((ntt_pass_kernel<ELOG, PM>), dim3((unsigned)blocks), dim3(threads), lds_bytes, stream, y, x,
-                       tw, L, S, k, Glog, klds, total_groups, md);
+                       tw, L, S, k, Glog, klds, total_groups, md, done, seq);
     return hipGetLastError();
 }
 
 template <bool PM>
 static hipError_t launch_pass_e(U128 *y, const U128 *x, const U128 *tw, int L, int S, int k, int Glog, int elog,
-                                size_t count, const Mod &md, hipStream_t stream)
+                                size_t count, const Mod &md, hipStream_t stream, uint32_t *done = nullptr,
+                                uint32_t seq = 0)
 {
     switch (elog) {
     case 1:
-        return launch_pass<1, PM>(y, x, tw, L, S, k, Glog, count, md, stream);
+        return launch_pass<1, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq);
     case 2:
-        return launch_pass<2, PM>(y, x, tw, L, S, k, Glog, count, md, stream);
+        return launch_pass<2, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq);
     default:
-        return launch_pass<3, PM>(y, x, tw, L, S, k, Glog, count, md, stream);
+        return launch_pass<3, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq);
     }
 }
 
@@ -147,10 +158,13 @@ static hipError_t launch_pass_e(U128 *y, const U128 *x, const U128 *tw, int L, i
 constexpr size_t kFewTransforms = 64;
 
 hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, size_t count, const Mod &md, U128 *tmp,
-                              hipStream_t stream)
+                              hipStream_t stream, uint32_t *done, uint32_t seq)
 {
     if (count == 0) return hipSuccess;
     if (L < kMinLog2 || L > kMaxLog2) return hipErrorInvalidValue;
+    // a completion flag needs a launch of one workgroup: one transform of at
+    // most 4096 points
+    if (done && (count != 1 || L > kLdsMaxLog2)) return hipErrorInvalidValue;
     const bool pm = md.pm != 0;
     if (L <= kLdsMaxLog2) {
         int elog, Glog;
@@ -164,8 +178,8 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
             Glog = 8 + elog - L;
             if (Glog < 0) Glog = 0;
         }
-        return pm ? launch_pass_e<true>(y, x, tw, L, 0, L, Glog, elog, count, md, stream)
-                  : launch_pass_e<false>(y, x, tw, L, 0, L, Glog, elog, count, md, stream);
+        return pm ? launch_pass_e<true>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq)
+                  : launch_pass_e<false>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq);
     }
     if (tmp == nullptr) return hipErrorInvalidValue;
     const int np = (L + 7) / 8;

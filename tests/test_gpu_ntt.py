@@ -38,6 +38,25 @@ def test_golden_host_entry(gpu, fixtures, n):
     assert np.array_equal(y, fixtures[f"ver_{n}"])
 
 
+@pytest.mark.parametrize("L", list(range(1, 13)))
+def test_host_entry_sizes(gpu, orc, L):
+    """Host-array calls at every one-workgroup size (the kernel posts a
+    completion word to pinned memory, which the call waits on): consecutive
+    calls with fresh inputs each match the oracle, the last with a value above
+    q (the literal formulas)."""
+    import benchntt_amd as bn
+
+    n = 1 << L
+    tw = uniform(n, QN, 300 + L)
+    for r in range(6):
+        x = uniform(n, QN, 310 + 16 * L + r)
+        if r == 5:
+            x[0] = [2**64 - 1, 2**64 - 1]
+        y = np.zeros_like(x)
+        bn.ntt(y, x, QN, MUN, tw)
+        assert np.array_equal(y, orc.ntt(x, tw, QN, MUN)), r
+
+
 @pytest.mark.parametrize("L", list(range(1, 17)))
 def test_random_batches(gpu, orc, L):
     """Every size 2^1..2^16, a few transforms per launch (one workgroup may
