@@ -501,9 +501,7 @@ def serving(torch, np, n_req: int = 16384) -> dict:
         return lat, span, st
 
     zero = np.zeros(n_req, dtype=np.uint64)
-    # untimed warm-up: one closed loop and one Poisson trace (the first trace
-    # of a process has shown a one-off host stall of about 7 ms in the
-    # scheduler loop, outside every HIP call)
+    # untimed warm-up: one closed loop and one Poisson trace
     trace(zero)
     trace(np.cumsum(np.random.default_rng(9).exponential(1e9 / 3e5, size=n_req)).astype(np.uint64))
     lat, span, st = trace(zero)
