@@ -402,7 +402,10 @@ int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, siz
 
 size_t bntt_ntt_tmp_bytes(int log2n, size_t count)
 {
-    if (log2n <= bntt::kLdsMaxLog2 || log2n > bntt::kMaxLog2) return 0;
+    if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2) return 0;
+    // one pass on one workgroup per transform, except a lone transform of
+    // 2^11 or 2^12 points, which with a tmp takes two passes over several CUs
+    if (log2n <= bntt::kLdsMaxLog2 && !(count == 1 && log2n >= bntt::kSpreadMinLog2)) return 0;
     if (count > (SIZE_MAX / sizeof(bntt_u128)) >> log2n) return SIZE_MAX;  // no such buffer exists
     return (count << log2n) * sizeof(bntt_u128);
 }
@@ -430,6 +433,14 @@ int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t c
         // before that transform is loaded
         return fail(BNTT_EINVAL, "y must equal x or not overlap it");
     }
+    if (bntt_ntt_tmp_bytes(log2n, count) == 0) {
+        tmp = nullptr;  // one pass: a tmp given anyway is not touched
+    } else {
+        const char *tb = static_cast<const char *>(tmp), *yb = static_cast<const char *>(y),
+                   *xb = static_cast<const char *>(x);
+        if ((tb < xb + bytes && xb < tb + bytes) || (tb < yb + bytes && yb < tb + bytes))
+            return fail(BNTT_EINVAL, "tmp must not overlap x or y");
+    }
     const Mod md = make_mod(m, mu);
     hipError_t e = bntt::launch_ntt_uniform(static_cast<U128 *>(y), static_cast<const U128 *>(x),
                                             static_cast<const U128 *>(tw), log2n, count, md,
@@ -446,7 +457,7 @@ int bntt_ntt_fwd_dev_noalloc(void *y, const void *x, const void *tw, int log2n, 
     if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2)
         return fail(BNTT_EINVAL, "log2n = %d outside [%d, %d]", log2n, bntt::kMinLog2, bntt::kMaxLog2);
     void *tmp = nullptr;
-    if (log2n > bntt::kLdsMaxLog2 && count > 0) {
+    if (count > 0 && bntt_ntt_tmp_bytes(log2n, count) != 0) {
         // multi-pass sizes keep their intermediates in the fixed scratch array;
         // every element of it that a later pass reads is written by an earlier
         // pass of the same call, so nothing needs clearing

@@ -852,7 +852,10 @@ int bntt_ntt_launch_each_dev(const bntt_ntt_task *tasks, size_t ntasks, const bn
             return bntt_set_error(BNTT_EINVAL, "tasks above 2^12 points need tmp");
         const hipError_t e = bntt::launch_ntt_uniform(static_cast<U128 *>(t.y), static_cast<const U128 *>(t.x),
                                                       static_cast<const U128 *>(t.tw), t.log2n, 1, md,
-                                                      static_cast<U128 *>(tmp), static_cast<hipStream_t>(stream));
+                                                      // one launch for every one-pass size
+                                                      t.log2n > bntt::kLdsMaxLog2 ? static_cast<U128 *>(tmp)
+                                                                                  : nullptr,
+                                                      static_cast<hipStream_t>(stream));
         if (e != hipSuccess) return bntt_set_hip_error(e, "ntt kernel launch");
     }
     return BNTT_OK;

@@ -1,7 +1,8 @@
 // ntt.hip -- launch-per-call forward NTT: `count` transforms of one size with
-// one twiddle table, in one launch (N <= 4096) or ceil(log2(N)/8) launches
-// through device memory (N > 4096).  The network and its blocking are
-// described in ntt_core.hpp.
+// one twiddle table, in one launch (N <= 4096) or two or three launches
+// through device memory (N > 4096, and a lone transform of 2^11 or 2^12
+// points given a tmp).  The network and its blocking are described in
+// ntt_core.hpp.
 #include "internal.hpp"
 #include "ntt_core.hpp"
 
@@ -166,7 +167,11 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
     // most 4096 points
     if (done && (count != 1 || L > kLdsMaxLog2)) return hipErrorInvalidValue;
     const bool pm = md.pm != 0;
-    if (L <= kLdsMaxLog2) {
+    // A lone transform of 2^11 or 2^12 points on one workgroup is
+    // latency-bound (15 and 26 us); given a tmp, it takes two passes over
+    // several CUs instead.
+    const bool spread = tmp != nullptr && count == 1 && L >= kSpreadMinLog2 && done == nullptr;
+    if (L <= kLdsMaxLog2 && !spread) {
         int elog, Glog;
         if (count <= kFewTransforms) {
             // one transform per workgroup, as many threads as the network allows
@@ -182,17 +187,23 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
                   : launch_pass_e<false>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq);
     }
     if (tmp == nullptr) return hipErrorInvalidValue;
-    const int np = (L + 7) / 8;
+    const int np = L <= 16 ? 2 : 3;
+    // 2048 values per workgroup (radix-8 sub-passes) when a pass has at least
+    // 2048 such tiles; fewer values per workgroup below that, so that the
+    // passes of a lone transform spread over more CUs
+    const size_t total = count << L;
+    const int elog = total >= ((size_t)1 << 22) ? 3 : total >= ((size_t)1 << 21) ? 2 : 1;
+    const int tile_log = 8 + elog;
     int ks[3];
-    if (!pass_split(L, np, 3, 11, ks)) return hipErrorInvalidValue;
+    if (!pass_split(L, np, elog, tile_log, ks)) return hipErrorInvalidValue;
     int S = 0;
     for (int p = 0; p < np; p++) {
         const int k = ks[p];
-        const int Glog = 11 - k;  // 2048 values per workgroup
+        const int Glog = tile_log - k;
         const U128 *src = (p == 0) ? x : ((np - p) % 2 == 0 ? y : tmp);
         U128 *dst = ((np - 1 - p) % 2 == 0) ? y : tmp;
-        const hipError_t e = pm ? launch_pass_e<true>(dst, src, tw, L, S, k, Glog, 3, count, md, stream)
-                                : launch_pass_e<false>(dst, src, tw, L, S, k, Glog, 3, count, md, stream);
+        const hipError_t e = pm ? launch_pass_e<true>(dst, src, tw, L, S, k, Glog, elog, count, md, stream)
+                                : launch_pass_e<false>(dst, src, tw, L, S, k, Glog, elog, count, md, stream);
         if (e != hipSuccess) return e;
         S += k;
     }

@@ -86,6 +86,16 @@ def test_large_plain(gpu, orc, L):
     assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
 
 
+@pytest.mark.parametrize("L,count", [(13, 1), (16, 1), (19, 1), (21, 1), (14, 160), (13, 512)])
+def test_multi_pass_tile_sizes(gpu, orc, L, count):
+    """Multi-pass launches pick 512-, 1024- or 2048-value tiles by the values
+    in the launch (below 2^21, below 2^22, else), in two or three passes:
+    cases of each, against the oracle."""
+    n = 1 << L
+    x, tw = uniform(count * n, QN, 90 + L), uniform(n, QN, 91 + L)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
 def test_uniform_batch_8192x1024(gpu, orc):
     """The uniform-batch workload: 8192 transforms of 1024 points."""
     x, tw = uniform(8192 * 1024, QN, 3), uniform(1024, QN, 4)
@@ -206,3 +216,42 @@ def test_bad_sizes(gpu):
     tw3 = gpu.zeros((3, 2), dtype=gpu.int64, device="cuda")
     with pytest.raises(ValueError):
         bn.ntt(x[:3], x[:3], QN, MUN, tw3)
+
+
+@pytest.mark.parametrize("L", [11, 12])
+def test_lone_transform_two_passes(gpu, orc, L):
+    """A lone 2^11 or 2^12 transform given a tmp takes two passes over several
+    CUs: out of place and in place against the oracle, the same result
+    without a tmp (one workgroup), and a tmp overlapping x rejected."""
+    import ctypes
+
+    import benchntt_amd as bn
+    from benchntt_amd import _lib
+
+    n = 1 << L
+    x, tw = uniform(n, QN, 500 + L), uniform(n, QN, 501 + L)
+    want = orc.ntt(x, tw, QN, MUN)
+    lib = _lib.load()
+    dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+    tmp = gpu.empty((n, 2), dtype=gpu.int64, device="cuda")
+    m, mu = _lib.U128.of(QN), _lib.U128.of(MUN)
+
+    def run(y, xx, t):
+        _lib.check(lib.bntt_ntt_fwd_dev(y.data_ptr(), xx.data_ptr(), dtw.data_ptr(), L, 1, ctypes.byref(m),
+                                        ctypes.byref(mu), t.data_ptr() if t is not None else None, None))
+        gpu.cuda.synchronize()
+
+    dy = gpu.empty_like(dx)
+    run(dy, dx, tmp)
+    assert np.array_equal(from_dev(dy), want)
+    dz = dx.clone()
+    run(dz, dz, tmp)
+    assert np.array_equal(from_dev(dz), want)
+    dw = gpu.empty_like(dx)
+    run(dw, dx, None)
+    assert np.array_equal(from_dev(dw), want)
+    buf = gpu.empty((3 * n, 2), dtype=gpu.int64, device="cuda")
+    buf[:n] = dx
+    with pytest.raises(bn.BnttError) as ei:
+        run(buf[n:2 * n], buf[:n], buf[n // 2:n // 2 + n])
+    assert "tmp" in str(ei.value)
