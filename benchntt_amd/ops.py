@@ -25,7 +25,7 @@ from ._lib import U128
 
 
 def _is_torch(a) -> bool:
-    return type(a).__module__.startswith("torch")
+    return type(a) is not np.ndarray and type(a).__module__.startswith("torch")
 
 
 def _nelem(a) -> int:
@@ -47,9 +47,25 @@ def _dev_ptr(a) -> int:
 
 
 def _host_ptr(a: np.ndarray) -> int:
-    if not isinstance(a, np.ndarray) or not a.flags["C_CONTIGUOUS"]:
+    if not isinstance(a, np.ndarray) or not a.flags.c_contiguous:
         raise ValueError("host arrays must be C-contiguous numpy arrays")
+    if a.flags.writeable and a.nbytes:
+        # the buffer protocol gives the address faster than a.ctypes
+        return ctypes.addressof(ctypes.c_char.from_buffer(a))
     return a.ctypes.data
+
+
+_U128_CACHE: dict = {}
+
+
+def _u128(v: int) -> U128:
+    """The (read-only) U128 of a modulus or scalar, reused across calls."""
+    c = _U128_CACHE.get(v)
+    if c is None:
+        if len(_U128_CACHE) > 64:
+            _U128_CACHE.clear()
+        c = _U128_CACHE[v] = U128.of(v)
+    return c
 
 
 def _stream(stream=None):
@@ -77,8 +93,8 @@ def _same_len(*arrays) -> int:
 
 def _blas(name, c, a, b, extra_before_n, m, mu, stream):
     n = _same_len(c, a, b)
-    mm = U128.of(m)
-    args_mu = [ctypes.byref(U128.of(mu))] if mu is not None else []
+    mm = _u128(m)
+    args_mu = [ctypes.byref(_u128(mu))] if mu is not None else []
     if _mode(c, a, b) == "dev":
         ptrs = [_dev_ptr(c), _dev_ptr(a), _dev_ptr(b)]
         _lib.call(f"bntt_{name}_dev", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu, _stream(stream))
@@ -109,7 +125,9 @@ def axpy(c, a, b, x: int, m: int, mu: int, stream=None) -> None:
 
 
 def tmp_elements(log2n: int, count: int = 1) -> int:
-    """Residues of scratch the device NTT needs (0 for log2n <= 12)."""
+    """Residues of scratch the device NTT uses: 0 for log2n <= 12, except a
+    lone transform of 2^11 or 2^12 points (optional there: with it the
+    transform takes two passes over several CUs)."""
     return _lib.load().bntt_ntt_tmp_bytes(log2n, count) // 16
 
 
@@ -118,8 +136,9 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
     `twiddles` (N entries).  N = len(twiddles); X and Y hold a whole number of
     transforms of N points (one, as in the reference, or a uniform batch).
     as0/as1 are the reference's scratch arguments: on the device path as0
-    serves as the multi-pass scratch when N > 4096 (len >= len(X)); as1 is
-    accepted and unused."""
+    serves as the scratch when the launch uses one (N > 4096, len >= len(X);
+    a lone transform of 2^11 or 2^12 points, which then takes two passes); as1
+    is accepted and unused."""
     del as1
     n = _nelem(twiddles)
     log2n = n.bit_length() - 1
@@ -129,7 +148,7 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
     if total % n:
         raise ValueError("X must hold a whole number of transforms")
     count = total // n
-    mm, mmu = U128.of(modulus), U128.of(mu)
+    mm, mmu = _u128(modulus), _u128(mu)
     if _mode(Y, X, twiddles) == "dev":
         need = tmp_elements(log2n, count)
         tmp = 0
@@ -153,7 +172,7 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
 
             own.record_stream(torch.cuda.ExternalStream(s.value or 0, device=X.device))
     else:
+        py, px, pt = _host_ptr(Y), _host_ptr(X), _host_ptr(twiddles)
         for i in range(count):
             off = i * n * 16
-            _lib.call("bntt_ntt_fwd", _host_ptr(Y) + off, _host_ptr(X) + off, _host_ptr(twiddles), log2n,
-                      ctypes.byref(mm), ctypes.byref(mmu))
+            _lib.call("bntt_ntt_fwd", py + off, px + off, pt, log2n, ctypes.byref(mm), ctypes.byref(mmu))
