@@ -213,7 +213,8 @@ __device__ __forceinline__ void fetch_twiddles(R31 (&t)[(1 << KP) - 1], const Tw
 }
 
 // KP stages of the constant-geometry network on 2^KP values in registers with
-// fetched digit twiddles (fast exact reduction).
+// fetched digit twiddles (fast exact reduction; the modulus is of the fast
+// form, so the add and subtract take its upper limbs as constants).
 template <int KP>
 __device__ __forceinline__ void reg_network_fast(U128 (&v)[1 << KP], const R31 (&t)[(1 << KP) - 1], const Mod &md)
 {
@@ -224,8 +225,8 @@ __device__ __forceinline__ void reg_network_fast(U128 (&v)[1 << KP], const R31 (
 #pragma unroll
         for (int i = 0; i < H; i++) {
             const U128 p = mulmod_r31(v[i + H], t[(1 << rp) - 1 + (i & ((1 << rp) - 1))], md.c);
-            w[2 * i] = addmod(v[i], p, md);
-            w[2 * i + 1] = submod(v[i], p, md);
+            w[2 * i] = addmod_pm(v[i], p, md.m.v[0]);
+            w[2 * i + 1] = submod_pm(v[i], p, md.m.v[0]);
         }
 #pragma unroll
         for (int i = 0; i < 2 * H; i++) v[i] = w[i];

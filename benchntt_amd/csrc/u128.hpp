@@ -167,6 +167,35 @@ BNTT_HD U128 submod_ref(const U128 &a, const U128 &b, const U128 &m)
     return r;
 }
 
+// The same two formulas for a modulus of the fast form m = 2^124 - c (c < 2^16):
+// limbs 1..3 of m are 0xFFFFFFFF, 0xFFFFFFFF, 0x0FFFFFFF and those of ~m 0, 0,
+// 0xF0000000 for every such m, so they enter as constants (the first two as
+// inline operands, which an add with carry-in can take beside its carry,
+// where a register operand would first be copied into a vector register).
+// Same operations, same bits, for all inputs.
+BNTT_HD U128 addmod_pm(const U128 &a, const U128 &b, uint32_t m0)
+{
+    U128 m, notm;
+    m.v[0] = m0;
+    m.v[1] = 0xFFFFFFFFu;
+    m.v[2] = 0xFFFFFFFFu;
+    m.v[3] = 0x0FFFFFFFu;
+    notm.v[0] = ~m0;
+    notm.v[1] = 0u;
+    notm.v[2] = 0u;
+    notm.v[3] = 0xF0000000u;
+    return addmod_ref(a, b, m, notm);
+}
+BNTT_HD U128 submod_pm(const U128 &a, const U128 &b, uint32_t m0)
+{
+    U128 m;
+    m.v[0] = m0;
+    m.v[1] = 0xFFFFFFFFu;
+    m.v[2] = 0xFFFFFFFFu;
+    m.v[3] = 0x0FFFFFFFu;
+    return submod_ref(a, b, m);
+}
+
 // ------------------------------------------------------------------- products
 
 // Full 256-bit product of two 128-bit values (schoolbook, 16 multiply-adds).

@@ -14,7 +14,8 @@ extern "C" {
 //     6 mulmod_r31 (b converted to digits first), 7 montmul_r31 (mu = -q^-1 mod 2^124),
 //     8 addmod_canon, 9 submod_canon, 10 montmul_r31_lazy (mu = -q^-1 mod 2^124),
 //     11 one lazy Montgomery butterfly of x = a, y = b with w = c (out[2i], out[2i+1]),
-//     12 montmul_r31_lazy<SPARSE> (q's 31-bit digits 1 and 2 zero)
+//     12 montmul_r31_lazy<SPARSE> (q's 31-bit digits 1 and 2 zero),
+//     13 addmod_pm, 14 submod_pm (m of the form 2^124 - c: only its limb 0 is read)
 void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U128 *m, const U128 *mu, uint32_t c)
 {
     bntt::Mod md;
@@ -41,6 +42,8 @@ void chk_op(int op, U128 *out, const U128 *a, const U128 *b, size_t n, const U12
             break;
         }
         case 8: out[i] = bntt::addmod_canon(a[i], b[i], *m); break;
+        case 13: out[i] = bntt::addmod_pm(a[i], b[i], m->v[0]); break;
+        case 14: out[i] = bntt::submod_pm(a[i], b[i], m->v[0]); break;
         case 10:
         case 11:
         case 12: {

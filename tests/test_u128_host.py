@@ -69,6 +69,10 @@ def test_device_arith_matches_oracle(chk, orc, q, mu):
     small = (a[:, 1] >> np.uint64(60) == 0) & (b[:, 1] >> np.uint64(60) == 0)
     assert np.array_equal(run(chk, 3, a[small], b[small], q, mu, c), want[small])
     assert np.array_equal(run(chk, 6, a[small], b[small], q, mu, c), want[small])
+    # the fast-form add/sub (q's limbs 1..3 as constants): the reference's bits
+    # for every operand, the addmod edge and operands >= 2^124 included
+    assert np.array_equal(run(chk, 13, a, b, q, mu), orc.vvadd(a, b, q))
+    assert np.array_equal(run(chk, 14, a, b, q, mu), orc.vvsub(a, b, q))
 
 
 def test_device_arith_other_modulus(chk, orc):
