@@ -186,7 +186,7 @@ struct bntt_pm_server {
     int grid_cap = 0;         // resident workgroups with up to kQueueLds3 of dynamic LDS
     int grid_cap2 = 0;        // ... with up to kQueueMaxLds
     size_t tw_budget = 0;     // twiddles from device memory (staging measured slower)
-    bool sparse = false;      // q's 31-bit digits 1 and 2 are zero: the shorter Montgomery reduction
+    bool sparse = false;      // q < 2^121 with 31-bit digits 1 and 2 zero: the shorter Montgomery reduction
 };
 
 namespace {
@@ -553,7 +553,9 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
             destroy_server(s);
             return rc;
         }
-    s->sparse = s->md.mq.q.d[1] == 0 && s->md.mq.q.d[2] == 0;
+    // (the sparse product sums a column of the product and of the reduction
+    // in one 64-bit accumulator, which needs q < 2^121: digit 3 below 2^28)
+    s->sparse = s->md.mq.q.d[1] == 0 && s->md.mq.q.d[2] == 0 && s->md.mq.q.d[3] < (1u << 28);
     const void *kfn = s->sparse ? reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT_S>)
                                 : reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT>);
     const void *kfn8 = s->sparse ? reinterpret_cast<const void *>(&bntt::queue_kernel<bntt::QK_MONT_S8>)

@@ -387,6 +387,58 @@ BNTT_HD U128 montmul_r31_lazy(const U128 &a_, const R31 &w, const MontQ &mq)
 {
     const R31 a = to_r31(a_);
     uint64_t t;
+    if constexpr (SPARSE) {
+        // q's digits 1 and 2 are zero, so a column holds at most four digit
+        // products of a*w and two of m*q: the reduction runs inside the
+        // product, column by column, every sum a 64-bit multiply-add on the
+        // running column (no digit extracted between the product and the
+        // reduction).  Bound, for q < 2^121, a < 4q, w < q, D = 2^31 - 1: the
+        // fullest column (digit 3) is below 3 D^2 + 6 D 2^28 + 2^34 < 2^64.
+        // m_i clears digit i of the running sum, as below; m is the same
+        // unique m, so are the bits.  The caller selects SPARSE only for
+        // q < 2^121.
+        const R31 &q = mq.q;
+        const uint32_t qi0 = mq.qinv.d[0];
+        uint32_t m0, m1, m2, m3;
+        t = (uint64_t)a.d[0] * w.d[0];
+        m0 = ((uint32_t)t * qi0) & kM31;
+        t = mad32(m0, q.d[0], t);
+        t = mad32(a.d[0], w.d[1], t >> 31);
+        t = mad32(a.d[1], w.d[0], t);
+        m1 = ((uint32_t)t * qi0) & kM31;
+        t = mad32(m1, q.d[0], t);
+        t = mad32(a.d[0], w.d[2], t >> 31);
+        t = mad32(a.d[1], w.d[1], t);
+        t = mad32(a.d[2], w.d[0], t);
+        m2 = ((uint32_t)t * qi0) & kM31;
+        t = mad32(m2, q.d[0], t);
+        t = mad32(a.d[0], w.d[3], t >> 31);
+        t = mad32(a.d[1], w.d[2], t);
+        t = mad32(a.d[2], w.d[1], t);
+        t = mad32(a.d[3], w.d[0], t);
+        t = mad32(m0, q.d[3], t);
+        m3 = ((uint32_t)t * qi0) & kM31;
+        t = mad32(m3, q.d[0], t);
+        t = mad32(a.d[1], w.d[3], t >> 31);
+        t = mad32(a.d[2], w.d[2], t);
+        t = mad32(a.d[3], w.d[1], t);
+        t = mad32(m1, q.d[3], t);
+        const uint32_t r0 = (uint32_t)t & kM31;
+        t = mad32(a.d[2], w.d[3], t >> 31);
+        t = mad32(a.d[3], w.d[2], t);
+        t = mad32(m2, q.d[3], t);
+        const uint32_t r1 = (uint32_t)t & kM31;
+        t = mad32(a.d[3], w.d[3], t >> 31);
+        t = mad32(m3, q.d[3], t);
+        const uint32_t r2 = (uint32_t)t & kM31;
+        const uint32_t r3 = (uint32_t)(t >> 31);  // u/R < 2q < 2^123
+        U128 u;
+        u.v[0] = r0 | (r1 << 31);
+        u.v[1] = (r1 >> 1) | (r2 << 30);
+        u.v[2] = (r2 >> 2) | (r3 << 29);
+        u.v[3] = r3 >> 3;
+        return u;
+    }
     uint32_t d[8];
     t = (uint64_t)a.d[0] * w.d[0];
     d[0] = (uint32_t)t & kM31;

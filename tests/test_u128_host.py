@@ -111,24 +111,29 @@ def test_lazy_montgomery(chk):
     4q and a canonical twiddle the product is below 2q and congruent to
     a*w*2^-124; the butterfly's outputs are below 4q and congruent to
     x + y*w*2^-124 and x - y*w*2^-124.  Also the exact product on inputs up to
-    4q.  Moduli: q_116, the largest prime below 2^122 with 2^17 | q - 1, and 97;
-    for q_116 and 97, whose middle 31-bit digits are zero, the sparse reduction
-    gives the same bits."""
+    4q.  Moduli: q_116, the largest prime below 2^122 with 2^17 | q - 1, 97,
+    and the largest odd q < 2^121 with zero middle digits; for those whose
+    middle 31-bit digits are zero and q < 2^121 (the sparse kernel's range,
+    where a column of product and reduction fits one 64-bit sum), the sparse
+    reduction gives the same bits, also on operands whose digits are all at
+    their maximum."""
     from benchntt_amd.residues import to_ints
 
     R = 1 << 124
-    for q in (C.Q_116, (1 << 122) - 6160383, 97):
+    for q in (C.Q_116, (1 << 122) - 6160383, 97, (((1 << 28) - 1) << 93) + (1 << 31) - 1):
         qinv = (-pow(q, -1, R)) % R
         rinv = pow(R, -1, q)
         rng = random.Random(q)
         edges = [0, 1, q - 1, q, 2 * q - 1, 2 * q, 4 * q - 1, 3 * q + 5]
+        # operands whose 31-bit digits are (nearly) all maximal below 4q
+        edges += [v for v in ((1 << 124) - 1, (1 << 93) - 1, ((1 << 124) - 1) >> 2) if v < 4 * q]
         av = [e for e in edges for _ in edges] + [rng.randrange(4 * q) for _ in range(5000)]
         bv = edges * len(edges) + [rng.randrange(q) for _ in range(5000)]
         bv = [y % q for y in bv]  # the twiddle operand is canonical
         a, b = from_ints(av), from_ints(bv)
         lazy = to_ints(run(chk, 10, a, b, q, qinv))
         assert all(v < 2 * q and v % q == x * y * rinv % q for v, x, y in zip(lazy, av, bv))
-        if (q >> 31) % (1 << 62) == 0:  # 31-bit digits 1 and 2 of q are zero: the sparse reduction
+        if (q >> 31) % (1 << 62) == 0 and q < (1 << 121):  # the sparse kernel's moduli
             assert to_ints(run(chk, 12, a, b, q, qinv)) == lazy
         assert to_ints(run(chk, 7, a, b, q, qinv)) == [x * y * rinv % q for x, y in zip(av, bv)]
         # butterfly with a one-digit twiddle w < min(q, 2^31)
