@@ -212,8 +212,8 @@ struct Staging {
     void *hbuf = nullptr;  // pinned host memory
     void *hdev = nullptr;  // its address for kernels
     size_t hcap = 0;
-    uint32_t *flag_h = nullptr;  // pinned completion word of one-workgroup launches
-    uint32_t *flag_d = nullptr;  // its address for kernels
+    uint32_t *flag_h = nullptr;  // pinned completion words (one per workgroup of a launch)
+    uint32_t *flag_d = nullptr;  // their address for kernels
     uint32_t seq = 0;            // last sequence number posted to it
 };
 std::mutex g_stages_mu;
@@ -258,7 +258,7 @@ int stage_reserve(Staging &stg, size_t bytes)
     return BNTT_OK;
 }
 
-// The pinned completion word of the staging (allocated at first use).
+// The pinned completion words of the staging (allocated at first use).
 int stage_flag(Staging &stg)
 {
     if (stg.flag_h) return BNTT_OK;
@@ -272,7 +272,8 @@ int stage_flag(Staging &stg)
     }
     stg.flag_h = static_cast<uint32_t *>(h);
     stg.flag_d = static_cast<uint32_t *>(d);
-    __atomic_store_n(stg.flag_h, stg.seq, __ATOMIC_RELEASE);
+    // 16 words: one per workgroup of a launch that posts completion words
+    for (int w = 0; w < 16; w++) __atomic_store_n(stg.flag_h + w, stg.seq, __ATOMIC_RELEASE);
     return BNTT_OK;
 }
 
@@ -281,18 +282,25 @@ int stage_flag(Staging &stg)
 // stream's own completion, so it spins on the word, for at most about 2 ms;
 // only if the word has not arrived by then is the stream synchronised (which
 // also reports a failed launch).
-int wait_flag(Staging &stg, uint32_t seq)
+int wait_flag(Staging &stg, uint32_t seq, int words = 1)
 {
     using clk = std::chrono::steady_clock;
     const auto t0 = clk::now();
+    int w = 0;  // words [0, w) have arrived
     for (uint32_t i = 1;; i++) {
-        if (__atomic_load_n(stg.flag_h, __ATOMIC_ACQUIRE) == seq) return BNTT_OK;
+        while (w < words && __atomic_load_n(stg.flag_h + w, __ATOMIC_ACQUIRE) == seq) w++;
+        if (w == words) return BNTT_OK;
         if ((i & 255u) == 0 && clk::now() - t0 > std::chrono::milliseconds(2)) break;
     }
     const hipError_t e = hipStreamSynchronize(stg.stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    if (__atomic_load_n(stg.flag_h, __ATOMIC_ACQUIRE) != seq)
-        return fail(BNTT_EHIP, "the transform completed without posting its completion word");
+    for (; w < words; w++) {
+        const uint32_t v = __atomic_load_n(stg.flag_h + w, __ATOMIC_ACQUIRE);
+        if (v == (seq ^ 0x80000000u))
+            return fail(BNTT_EHIP, "workgroup %d of the transform gave up waiting for its siblings at the grid "
+                                   "barrier (the GPU did not run them within about a second); results discarded", w);
+        if (v != seq) return fail(BNTT_EHIP, "the transform completed without posting its completion word");
+    }
     return BNTT_OK;
 }
 
@@ -404,8 +412,12 @@ size_t bntt_ntt_tmp_bytes(int log2n, size_t count)
 {
     if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2) return 0;
     // one pass on one workgroup per transform, except a lone transform of
-    // 2^11 or 2^12 points, which with a tmp takes two passes over several CUs
-    if (log2n <= bntt::kLdsMaxLog2 && !(count == 1 && log2n >= bntt::kSpreadMinLog2)) return 0;
+    // 2^11 or 2^12 points, which with a tmp (its intermediate and 16 residues
+    // of grid-barrier words) splits over several workgroups
+    if (log2n <= bntt::kLdsMaxLog2) {
+        if (count != 1 || log2n < bntt::kLoneSplitMinLog2) return 0;
+        return (((size_t)1 << log2n) + 16) * sizeof(bntt_u128);
+    }
     if (count > (SIZE_MAX / sizeof(bntt_u128)) >> log2n) return SIZE_MAX;  // no such buffer exists
     return (count << log2n) * sizeof(bntt_u128);
 }
@@ -433,12 +445,13 @@ int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t c
         // before that transform is loaded
         return fail(BNTT_EINVAL, "y must equal x or not overlap it");
     }
-    if (bntt_ntt_tmp_bytes(log2n, count) == 0) {
+    const size_t tmpb = bntt_ntt_tmp_bytes(log2n, count);
+    if (tmpb == 0) {
         tmp = nullptr;  // one pass: a tmp given anyway is not touched
-    } else {
+    } else if (tmp) {
         const char *tb = static_cast<const char *>(tmp), *yb = static_cast<const char *>(y),
                    *xb = static_cast<const char *>(x);
-        if ((tb < xb + bytes && xb < tb + bytes) || (tb < yb + bytes && yb < tb + bytes))
+        if ((tb < xb + bytes && xb < tb + tmpb) || (tb < yb + bytes && yb < tb + tmpb))
             return fail(BNTT_EINVAL, "tmp must not overlap x or y");
     }
     const Mod md = make_mod(m, mu);
@@ -458,9 +471,12 @@ int bntt_ntt_fwd_dev_noalloc(void *y, const void *x, const void *tw, int log2n, 
         return fail(BNTT_EINVAL, "log2n = %d outside [%d, %d]", log2n, bntt::kMinLog2, bntt::kMaxLog2);
     void *tmp = nullptr;
     if (count > 0 && bntt_ntt_tmp_bytes(log2n, count) != 0) {
-        // multi-pass sizes keep their intermediates in the fixed scratch array;
-        // every element of it that a later pass reads is written by an earlier
-        // pass of the same call, so nothing needs clearing
+        // multi-pass sizes, and a lone transform of 2^11 or 2^12 points split
+        // over several workgroups, keep their intermediates in the fixed
+        // scratch array; every element of it that a later pass reads is written
+        // by an earlier pass of the same call, and the split's barrier words
+        // are matched against a token no earlier launch used, so nothing needs
+        // clearing
         if (count > (kNoallocResidues >> log2n))
             return fail(BNTT_EINVAL,
                         "count * 2^log2n = %zu * 2^%d exceeds the %zu residues of the no-allocation scratch "
@@ -511,20 +527,23 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
     rc = stage_reserve(stg, 3 * bytes + tmpb);
     if (rc) return rc;
     hipStream_t s = stg.stream;
-    if (tmpb == 0 && bytes <= kZeroCopyBytes) {
-        // one pass, one workgroup: the kernel reads x and the table once from
-        // pinned memory, writes y there once and posts its completion word
+    if (bytes <= kZeroCopyBytes) {
+        // one launch: the kernel reads x and the table once from pinned
+        // memory, writes y there once and each of its workgroups posts a
+        // completion word (a transform of 2^11 or 2^12 points splits over
+        // several workgroups, with its intermediate in the device arena)
         if ((rc = stage_host_reserve(stg, 3 * bytes)) || (rc = stage_flag(stg))) return rc;
         char *hb = static_cast<char *>(stg.hbuf), *hd = static_cast<char *>(stg.hdev);
         std::memcpy(hb, X, bytes);
         std::memcpy(hb + bytes, tw, bytes);
         const uint32_t seq = ++stg.seq;
+        int words = 1;
         const hipError_t e =
             bntt::launch_ntt_uniform(reinterpret_cast<U128 *>(hd + 2 * bytes), reinterpret_cast<const U128 *>(hd),
-                                     reinterpret_cast<const U128 *>(hd + bytes), log2n, 1, make_mod(m, mu), nullptr,
-                                     s, stg.flag_d, seq);
+                                     reinterpret_cast<const U128 *>(hd + bytes), log2n, 1, make_mod(m, mu),
+                                     tmpb ? static_cast<U128 *>(stg.buf) : nullptr, s, stg.flag_d, seq, &words);
         if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
-        if ((rc = wait_flag(stg, seq))) return rc;
+        if ((rc = wait_flag(stg, seq, words))) return rc;
         std::memcpy(Y, hb + 2 * bytes, bytes);
         return BNTT_OK;
     }

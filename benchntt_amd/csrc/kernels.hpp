@@ -18,16 +18,20 @@ hipError_t launch_blas(int op, U128 *c, const U128 *a, const U128 *b, U128 x, si
 // Forward constant-geometry NTT of `count` transforms of size 2^log2n, stored
 // contiguously, all with twiddle table tw (2^log2n entries).  tmp must hold
 // count * 2^log2n residues when log2n > kLdsMaxLog2 (multi-pass sizes); it may
-// be null otherwise, and with it a lone transform of 2^kSpreadMinLog2 ..
-// 2^kLdsMaxLog2 points takes two passes.  x and y may alias only when
-// log2n <= kLdsMaxLog2.  With
-// `done` (one transform of at most 2^kLdsMaxLog2 points: a single workgroup)
-// the kernel stores `seq` to *done, at system scope, after its outputs.
+// be null otherwise.  Given for one transform of 2^11 or 2^12 points, it must
+// hold 2^log2n + 16 residues (the intermediate and the grid-barrier words),
+// and the transform splits over several workgroups (or, in a launch being
+// captured into a graph, takes two launches).  x and y may
+// alias only when log2n <= kLdsMaxLog2.  With `done` (one transform of at
+// most 2^kLdsMaxLog2 points) each of the launch's *done_words workgroups w
+// stores `seq` to done[w], at system scope, after its outputs.
 hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int log2n, size_t count, const Mod &md,
-                              U128 *tmp, hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0);
+                              U128 *tmp, hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0,
+                              int *done_words = nullptr);
 
 constexpr int kLdsMaxLog2 = 12;   // transforms up to 4096 points run in one LDS-resident pass
-constexpr int kSpreadMinLog2 = 11;  // ... except a lone one from 2^11 points, given a tmp: two passes
+constexpr int kLoneSplitMinLog2 = 11;  // a lone transform from 2^11 points given a tmp splits over workgroups
+constexpr int kSpreadMinLog2 = 11;  // ... except a lone one from 2^11 points given a tmp under capture: two launches
 constexpr int kMinLog2 = 1;
 constexpr int kMaxLog2 = 24;
 

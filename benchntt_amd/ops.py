@@ -29,10 +29,7 @@ def _is_torch(a) -> bool:
 
 
 def _nelem(a) -> int:
-    if _is_torch(a):
-        nbytes = a.numel() * a.element_size()
-    else:
-        nbytes = a.nbytes
+    nbytes = a.nbytes  # numpy arrays and torch tensors alike
     if nbytes % 16:
         raise ValueError("residue arrays hold 16 bytes per element")
     return nbytes // 16
@@ -68,13 +65,33 @@ def _u128(v: int) -> U128:
     return c
 
 
-def _stream(stream=None):
+_raw_stream = None
+
+
+def _current_raw_stream(device_index: int) -> int:
+    """torch's current stream on the device, as a hipStream_t handle (torch's
+    own raw accessor where it has one: building a Stream object per call costs
+    several microseconds)."""
+    global _raw_stream
+    if _raw_stream is None:
+        import torch
+
+        f = getattr(torch._C, "_cuda_getCurrentRawStream", None)
+        _raw_stream = f if f is not None else (lambda d: torch.cuda.current_stream(d).cuda_stream)
+    return _raw_stream(device_index)
+
+
+def _stream(stream=None, like=None):
+    """The stream argument of a device entry point: `stream` (a torch stream
+    object or a raw hipStream_t handle), else torch's current stream on the
+    device of tensor `like` (or the current device)."""
     if stream is not None:
-        # a torch stream object or a raw hipStream_t handle
         return ctypes.c_void_p(int(getattr(stream, "cuda_stream", stream)))
+    if like is not None:
+        return ctypes.c_void_p(_current_raw_stream(like.get_device()))
     import torch
 
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    return ctypes.c_void_p(_current_raw_stream(torch.cuda.current_device()))
 
 
 def _mode(*arrays) -> str:
@@ -97,7 +114,7 @@ def _blas(name, c, a, b, extra_before_n, m, mu, stream):
     args_mu = [ctypes.byref(_u128(mu))] if mu is not None else []
     if _mode(c, a, b) == "dev":
         ptrs = [_dev_ptr(c), _dev_ptr(a), _dev_ptr(b)]
-        _lib.call(f"bntt_{name}_dev", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu, _stream(stream))
+        _lib.call(f"bntt_{name}_dev", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu, _stream(stream, c))
     else:
         ptrs = [_host_ptr(c), _host_ptr(a), _host_ptr(b)]
         _lib.call(f"bntt_{name}", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu)
@@ -124,11 +141,20 @@ def axpy(c, a, b, x: int, m: int, mu: int, stream=None) -> None:
     _blas("axpy", c, a, b, [ctypes.byref(xs)], m, mu, stream)
 
 
+_TMP_CACHE: dict = {}
+
+
 def tmp_elements(log2n: int, count: int = 1) -> int:
     """Residues of scratch the device NTT uses: 0 for log2n <= 12, except a
-    lone transform of 2^11 or 2^12 points (optional there: with it the
-    transform takes two passes over several CUs)."""
-    return _lib.load().bntt_ntt_tmp_bytes(log2n, count) // 16
+    lone transform of 2^11 or 2^12 points (optional there, 2^log2n + 16
+    residues: with it the transform splits over several workgroups)."""
+    key = (log2n, count)
+    v = _TMP_CACHE.get(key)
+    if v is None:
+        if len(_TMP_CACHE) > 256:
+            _TMP_CACHE.clear()
+        v = _TMP_CACHE[key] = _lib.load().bntt_ntt_tmp_bytes(log2n, count) // 16
+    return v
 
 
 def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) -> None:
@@ -136,8 +162,9 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
     `twiddles` (N entries).  N = len(twiddles); X and Y hold a whole number of
     transforms of N points (one, as in the reference, or a uniform batch).
     as0/as1 are the reference's scratch arguments: on the device path as0
-    serves as the scratch when the launch uses one (N > 4096, len >= len(X);
-    a lone transform of 2^11 or 2^12 points, which then takes two passes); as1
+    serves as the scratch when the launch uses one (N > 4096: len >= len(X);
+    a lone transform of 2^11 or 2^12 points, which then splits over several
+    workgroups: len >= N + 16, else a scratch of that size is allocated); as1
     is accepted and unused."""
     del as1
     n = _nelem(twiddles)
@@ -154,16 +181,18 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
         tmp = 0
         own = None
         if need:
+            if as0 is not None and _nelem(as0) < need:
+                if log2n > 12:
+                    raise ValueError(f"as0 must hold {need} residues for this N")
+                as0 = None  # optional scratch too small for the split (N + 16): use our own
             if as0 is None:
                 import torch
 
                 as0 = own = torch.empty((need, 2), dtype=torch.int64, device=X.device)
-            if _nelem(as0) < need:
-                raise ValueError(f"as0 must hold {need} residues for this N")
             tmp = _dev_ptr(as0)
-        s = _stream(stream)
-        _lib.call("bntt_ntt_fwd_dev", _dev_ptr(Y), _dev_ptr(X), _dev_ptr(twiddles), log2n, count,
-                  ctypes.byref(mm), ctypes.byref(mmu), tmp, s)
+        py, px, pt = _dev_ptr(Y), _dev_ptr(X), _dev_ptr(twiddles)
+        s = _stream(stream, X)
+        _lib.call("bntt_ntt_fwd_dev", py, px, pt, log2n, count, ctypes.byref(mm), ctypes.byref(mmu), tmp, s)
         if own is not None and stream is not None:
             # the scratch was allocated on torch's current stream but is
             # written on `stream`: keep the allocator from reusing it before

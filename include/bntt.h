@@ -88,12 +88,14 @@ int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, siz
  * device memory: tmp must point to bntt_ntt_tmp_bytes(log2n, count) bytes of
  * device memory and y must not overlap x.  count = 1 is the plain
  * one-transform path; for log2n 11 and 12 with count = 1, tmp is optional
- * (bntt_ntt_tmp_bytes bytes): given, the transform takes two passes over
- * several CUs instead of one workgroup.  tmp must not overlap x or y. */
+ * (bntt_ntt_tmp_bytes bytes, 2^log2n + 16 residues): given, the transform
+ * splits over 4 or 8 workgroups joined by a grid barrier in one launch (in a
+ * launch being captured into a graph: two launches).  tmp must not overlap x or y, and a call that
+ * uses it must not run concurrently with another call using the same tmp. */
 int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t count, const bntt_u128 *m,
                      const bntt_u128 *mu, void *tmp, void *stream);
 /* Bytes of tmp for bntt_ntt_fwd_dev: 0 when log2n <= 12 (no tmp needed),
- * except count = 1 at log2n 11 and 12 (an optional tmp of 2^log2n
+ * except count = 1 at log2n 11 and 12 (an optional tmp of 2^log2n + 16
  * residues), SIZE_MAX when count * 2^log2n * 16 does not fit a size_t. */
 size_t bntt_ntt_tmp_bytes(int log2n, size_t count);
 

@@ -219,39 +219,89 @@ def test_bad_sizes(gpu):
 
 
 @pytest.mark.parametrize("L", [11, 12])
-def test_lone_transform_two_passes(gpu, orc, L):
-    """A lone 2^11 or 2^12 transform given a tmp takes two passes over several
-    CUs: out of place and in place against the oracle, the same result
-    without a tmp (one workgroup), and a tmp overlapping x rejected."""
+def test_lone_transform_split(gpu, orc, L):
+    """A lone 2^11 or 2^12 transform given a tmp (2^L + 16 residues) splits
+    over 4 or 8 workgroups joined by a grid barrier: out of place,
+    in place and three calls in a row on one tmp (each launch's barrier token
+    differs from the stale words of the last) against the oracle, the same
+    result without a tmp (one workgroup), with an input above q and with a
+    pass-2 twiddle >= 2^124 (literal formulas on some wavefronts), over a
+    modulus outside the fast form, under graph capture (which takes the
+    unsplit paths) with replays on refilled inputs, and a tmp overlapping x
+    rejected."""
     import ctypes
 
     import benchntt_amd as bn
     from benchntt_amd import _lib
 
     n = 1 << L
-    x, tw = uniform(n, QN, 500 + L), uniform(n, QN, 501 + L)
-    want = orc.ntt(x, tw, QN, MUN)
     lib = _lib.load()
-    dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
-    tmp = gpu.empty((n, 2), dtype=gpu.int64, device="cuda")
-    m, mu = _lib.U128.of(QN), _lib.U128.of(MUN)
+    need = lib.bntt_ntt_tmp_bytes(L, 1) // 16
+    assert need == n + 16
+    tmp = gpu.empty((need, 2), dtype=gpu.int64, device="cuda")
 
-    def run(y, xx, t):
+    def run(y, xx, dtw, t, q=QN, mu=MUN, stream=None):
+        m, u = _lib.U128.of(q), _lib.U128.of(mu)
         _lib.check(lib.bntt_ntt_fwd_dev(y.data_ptr(), xx.data_ptr(), dtw.data_ptr(), L, 1, ctypes.byref(m),
-                                        ctypes.byref(mu), t.data_ptr() if t is not None else None, None))
-        gpu.cuda.synchronize()
+                                        ctypes.byref(u), t.data_ptr() if t is not None else None, stream))
 
-    dy = gpu.empty_like(dx)
-    run(dy, dx, tmp)
-    assert np.array_equal(from_dev(dy), want)
+    for seed in range(3):
+        x, tw = uniform(n, QN, 500 + 7 * L + seed), uniform(n, QN, 501 + 7 * L + seed)
+        want = orc.ntt(x, tw, QN, MUN)
+        dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+        dy = gpu.empty_like(dx)
+        run(dy, dx, dtw, tmp)
+        gpu.cuda.synchronize()
+        assert np.array_equal(from_dev(dy), want), seed
     dz = dx.clone()
-    run(dz, dz, tmp)
-    assert np.array_equal(from_dev(dz), want)
+    run(dz, dz, dtw, tmp)
     dw = gpu.empty_like(dx)
-    run(dw, dx, None)
+    run(dw, dx, dtw, None)
+    gpu.cuda.synchronize()
+    assert np.array_equal(from_dev(dz), want)
     assert np.array_equal(from_dev(dw), want)
+
+    # literal formulas: an input above q (pass 1 of one wavefront, then pass 2
+    # of every wavefront that reads its outputs) and a table entry >= 2^124
+    # that only the last stage reads
+    xe, twe = x.copy(), tw.copy()
+    xe[5] = [2**64 - 1, 2**64 - 1]
+    twe[n - 3, 1] |= np.uint64(1 << 62)
+    dye = gpu.empty_like(dx)
+    run(dye, to_dev(gpu, xe), to_dev(gpu, twe), tmp)
+    gpu.cuda.synchronize()
+    assert np.array_equal(from_dev(dye), orc.ntt(xe, twe, QN, MUN))
+    q, mu = C.Q_116, (1 << 235) // C.Q_116
+    xq, twq = uniform(n, q, 600 + L), uniform(n, q, 601 + L)
+    dyq = gpu.empty_like(dx)
+    run(dyq, to_dev(gpu, xq), to_dev(gpu, twq), tmp, q, mu)
+    gpu.cuda.synchronize()
+    assert np.array_equal(from_dev(dyq), orc.ntt(xq, twq, q, mu))
+
+    # captured: replays follow refilled inputs
+    s = gpu.cuda.Stream()
+    dg, dyg = gpu.empty_like(dx), gpu.empty_like(dx)
+    dg.copy_(dx)
+    gpu.cuda.synchronize()
+    g = gpu.cuda.CUDAGraph()
+    with gpu.cuda.graph(g, stream=s):
+        run(dyg, dg, dtw, tmp, stream=ctypes.c_void_p(s.cuda_stream))
+    for r in range(3):
+        xr = uniform(n, QN, 700 + 5 * L + r)
+        dg.copy_(to_dev(gpu, xr))
+        gpu.cuda.synchronize()
+        g.replay()
+        gpu.cuda.synchronize()
+        assert np.array_equal(from_dev(dyg), orc.ntt(xr, tw, QN, MUN)), r
+    del g
+
     buf = gpu.empty((3 * n, 2), dtype=gpu.int64, device="cuda")
     buf[:n] = dx
     with pytest.raises(bn.BnttError) as ei:
-        run(buf[n:2 * n], buf[:n], buf[n // 2:n // 2 + n])
+        run(buf[n:2 * n], buf[:n], dtw, buf[n // 2:n // 2 + need])
     assert "tmp" in str(ei.value)
+    # the tmp's barrier words after its intermediate count as part of it
+    big = gpu.empty((2 * n + 16, 2), dtype=gpu.int64, device="cuda")
+    big[n + 8:2 * n + 8] = dx
+    with pytest.raises(bn.BnttError):
+        run(gpu.empty_like(dx), big[n + 8:2 * n + 8], dtw, big[:need])
