@@ -44,12 +44,18 @@ def _dev_ptr(a) -> int:
 
 
 def _host_ptr(a: np.ndarray) -> int:
-    if not isinstance(a, np.ndarray) or not a.flags.c_contiguous:
+    if not isinstance(a, np.ndarray):
         raise ValueError("host arrays must be C-contiguous numpy arrays")
-    if a.flags.writeable and a.nbytes:
+    fl = a.flags
+    if not fl.c_contiguous:
+        raise ValueError("host arrays must be C-contiguous numpy arrays")
+    if fl.writeable and a.nbytes:
         # the buffer protocol gives the address faster than a.ctypes
-        return ctypes.addressof(ctypes.c_char.from_buffer(a))
+        return ctypes.addressof(_c_char.from_buffer(a))
     return a.ctypes.data
+
+
+_c_char = ctypes.c_char
 
 
 _U128_CACHE: dict = {}
@@ -157,6 +163,39 @@ def tmp_elements(log2n: int, count: int = 1) -> int:
     return v
 
 
+_fwd_host = None
+
+
+def _ntt_host(Y, X, tw, modulus, mu) -> None:
+    """ntt() on numpy arrays: one synchronous host-array call per transform
+    (the same checks as the general path, in fewer Python steps)."""
+    global _fwd_host
+    nb = tw.nbytes
+    total = X.nbytes
+    if nb % 16 or total % 16:
+        raise ValueError("residue arrays hold 16 bytes per element")
+    if Y.nbytes != total:
+        raise ValueError("arrays differ in length")
+    n = nb >> 4
+    log2n = n.bit_length() - 1
+    if n != 1 << log2n:
+        raise ValueError("N must be a power of two")
+    if (total >> 4) % n:
+        raise ValueError("X must hold a whole number of transforms")
+    count = (total >> 4) // n
+    py, px, pt = _host_ptr(Y), _host_ptr(X), _host_ptr(tw)
+    if _fwd_host is None:
+        _fwd_host = _lib.load().bntt_ntt_fwd
+    mm, mmu = ctypes.byref(_u128(modulus)), ctypes.byref(_u128(mu))
+    step = n << 4
+    for i in range(count):
+        rc = _fwd_host(py, px, pt, log2n, mm, mmu)
+        if rc:
+            _lib.check(rc)
+        py += step
+        px += step
+
+
 def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) -> None:
     """Forward NTT (ntt_scalar.c:582-627): Y = NTT(X) with the twiddle table
     `twiddles` (N entries).  N = len(twiddles); X and Y hold a whole number of
@@ -167,6 +206,9 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
     workgroups: len >= N + 16, else a scratch of that size is allocated); as1
     is accepted and unused."""
     del as1
+    if type(Y) is np.ndarray and type(X) is np.ndarray and type(twiddles) is np.ndarray:
+        _ntt_host(Y, X, twiddles, modulus, mu)
+        return
     n = _nelem(twiddles)
     log2n = n.bit_length() - 1
     if n != 1 << log2n:

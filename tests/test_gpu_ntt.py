@@ -57,6 +57,34 @@ def test_host_entry_sizes(gpu, orc, L):
         assert np.array_equal(y, orc.ntt(x, tw, QN, MUN)), r
 
 
+def test_host_entry_table_changes(gpu, orc):
+    """Consecutive host-array calls whose twiddle tables change between
+    calls: the same table twice, another table, the first array modified in
+    place (one entry), tables of other sizes and back, and an entry >= 2^124
+    (the literal formulas), each call against the oracle."""
+    import benchntt_amd as bn
+
+    def call(x, tw):
+        y = np.zeros_like(x)
+        bn.ntt(y, x, QN, MUN, tw)
+        assert np.array_equal(y, orc.ntt(x, tw, QN, MUN))
+
+    n = 1024
+    tw1, tw2 = uniform(n, QN, 900), uniform(n, QN, 901)
+    x = uniform(n, QN, 902)
+    call(x, tw1)
+    call(uniform(n, QN, 903), tw1)
+    call(x, tw2)
+    call(x, tw1)
+    tw1[700] = uniform(1, QN, 904)[0]
+    call(x, tw1)
+    call(uniform(4096, QN, 905), uniform(4096, QN, 906))
+    call(uniform(16, QN, 907), tw1[:16].copy())
+    call(x, tw1)
+    tw1[1000, 1] |= np.uint64(1 << 62)
+    call(x, tw1)
+
+
 @pytest.mark.parametrize("L", list(range(1, 17)))
 def test_random_batches(gpu, orc, L):
     """Every size 2^1..2^16, a few transforms per launch (one workgroup may
