@@ -466,7 +466,8 @@ def uneven_batch(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
 def serving(torch, np, n_req: int = 16384) -> dict:
     """Negacyclic polynomial-multiply requests mod q_116 (forward NTT x2,
     pointwise product, inverse NTT) with N uniform on 2^10..2^16, served with
-    continuous batching under a 2 GiB HBM budget on 3 streams.  First a closed
+    continuous batching under a 2 GiB HBM budget on 3 streams, batches closed
+    at 64 requests or 24 M butterflies.  First a closed
     loop (every request arrived at t = 0) gives the saturation rate; then
     seeded Poisson arrivals at fractions of it give p50/p99 latency (arrival to
     completion seen by the host) and the achieved rate.  16384 requests per
@@ -489,9 +490,14 @@ def serving(torch, np, n_req: int = 16384) -> dict:
     torch.cuda.synchronize()
     budget = 2 << 30
     streams = 3
-    srv = PolymulServer(q, hbm_budget=budget, max_inflight=streams, max_batch=64, min_log2n=10, max_log2n=16)
+    # batches close at 64 requests or 24 M butterflies of work, whichever
+    # comes first (a batch's duration follows its work)
+    cap = 24_000_000
+    srv = PolymulServer(q, hbm_budget=budget, max_inflight=streams, max_batch=64, min_log2n=10, max_log2n=16,
+                        max_batch_butterflies=cap)
     srv.multiply(reqs[:64])  # warm-up
-    out = {"serving_requests": n_req, "serving_hbm_budget_bytes": budget,
+    out = {"serving_requests": n_req, "serving_hbm_budget_bytes": budget, "serving_max_batch": 64,
+           "serving_max_batch_butterflies": cap, "serving_streams": streams,
            "serving_mean_N": round(float(np.mean(sizes)), 1)}
 
     def trace(arrival):
@@ -534,7 +540,8 @@ def serving(torch, np, n_req: int = 16384) -> dict:
     # compared on the device with the unconstrained run's (outside the timing)
     ref_c = Cc.clone()
     tight = 128 << 20
-    srv_t = PolymulServer(q, hbm_budget=tight, max_inflight=streams, max_batch=64, min_log2n=10, max_log2n=16)
+    srv_t = PolymulServer(q, hbm_budget=tight, max_inflight=streams, max_batch=64, min_log2n=10, max_log2n=16,
+                          max_batch_butterflies=cap)
     srv_t.multiply(reqs[:4])
     Cc.zero_()
     gaps = np.random.default_rng(700).exponential(1e9 / (0.7 * sat), size=n_req)

@@ -78,7 +78,7 @@ class PmConfig(ctypes.Structure):
 
     _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_inflight", ctypes.c_int32),
                 ("max_batch", ctypes.c_int32), ("min_log2n", ctypes.c_int32), ("max_log2n", ctypes.c_int32),
-                ("admit_timeout_ns", ctypes.c_uint64)]
+                ("admit_timeout_ns", ctypes.c_uint64), ("max_batch_butterflies", ctypes.c_uint64)]
 
 
 class PmStats(ctypes.Structure):

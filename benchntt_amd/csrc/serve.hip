@@ -137,6 +137,9 @@ size_t request_units(int L)
     return ((bufs << L) * sizeof(U128) + kUnit - 1) / kUnit;
 }
 
+// Butterflies of one request: three transforms of N/2 * log2 N.
+uint64_t request_butterflies(int L) { return (uint64_t)3 * ((uint64_t)1 << (L - 1)) * (uint64_t)L; }
+
 // Upper bound of work items one request contributes (three transforms).
 size_t items_per_request(int L, int want)
 {
@@ -179,6 +182,7 @@ struct bntt_pm_server {
     HostMont hm;
     int min_l = 0, max_l = 0, max_batch = 0, nslots = 0;
     uint64_t admit_timeout_ns = 0;  // 0: requests wait for admission as long as it takes
+    uint64_t max_batch_bfly = 0;    // work cap of a trace batch in butterflies (0: none)
     SizeTables tabs[bntt::kMaxLog2 + 1];
     char *arena_d = nullptr;
     Arena arena;
@@ -535,6 +539,7 @@ int bntt_pm_server_create(bntt_pm_server **out, const bntt_u128 *q, const bntt_p
     s->max_batch = cfg->max_batch;
     s->nslots = cfg->max_inflight;
     s->admit_timeout_ns = cfg->admit_timeout_ns;
+    s->max_batch_bfly = cfg->max_batch_butterflies;
     for (int L = s->min_l; L <= s->max_l; L++)
         if ((rc = build_tables(s, L))) {
             destroy_server(s);
@@ -762,8 +767,13 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
             Slot &sl = s->slots[k];
             if (sl.busy) continue;
             views.clear();
+            uint64_t work = 0;  // butterflies of the batch so far
             while (!pending.empty() && (int)views.size() < s->max_batch) {
                 const size_t r = pending.front();
+                // a batch's duration follows its work: closing batches at a
+                // work cap keeps heavy ones from holding every request in them
+                const uint64_t w = request_butterflies(reqs[r].log2n);
+                if (s->max_batch_bfly && !views.empty() && work + w > s->max_batch_bfly) break;
                 const size_t u = request_units(reqs[r].log2n);
                 size_t off;
                 if (!s->arena.alloc(u, &off)) {
@@ -772,6 +782,7 @@ int bntt_pm_run_trace(bntt_pm_server *s, const bntt_pm_request *reqs, const uint
                     break;
                 }
                 pending.pop_front();
+                work += w;
                 sl.allocs.emplace_back(off, u);
                 sl.reqs.push_back(r);
                 views.push_back(ReqView{static_cast<const U128 *>(reqs[r].a), static_cast<const U128 *>(reqs[r].b),

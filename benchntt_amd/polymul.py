@@ -43,10 +43,14 @@ def _requests(reqs):
 
 class PolymulServer:
     def __init__(self, q: int, hbm_budget: int = 1 << 30, max_inflight: int = 4, max_batch: int = 64,
-                 min_log2n: int = 1, max_log2n: int = 16, admit_timeout_ns: int = 0):
+                 min_log2n: int = 1, max_log2n: int = 16, admit_timeout_ns: int = 0,
+                 max_batch_butterflies: int = 0):
         """admit_timeout_ns: run_trace drops a request not admitted within
-        this time of its arrival (0: never)."""
-        cfg = PmConfig(hbm_budget, max_inflight, max_batch, min_log2n, max_log2n, admit_timeout_ns)
+        this time of its arrival (0: never).  max_batch_butterflies: run_trace
+        closes a batch before a request that would take its work (3 * N/2 *
+        log2 N butterflies per request) above this (0: no limit)."""
+        cfg = PmConfig(hbm_budget, max_inflight, max_batch, min_log2n, max_log2n, admit_timeout_ns,
+                       max_batch_butterflies)
         h = ctypes.c_void_p()
         qq = U128.of(q)
         _lib.call("bntt_pm_server_create", ctypes.byref(h), ctypes.byref(qq), ctypes.byref(cfg))

@@ -238,6 +238,10 @@ typedef struct bntt_pm_config {
     int32_t max_log2n;         /* <= 16 */
     uint64_t admit_timeout_ns; /* bntt_pm_run_trace drops a request not admitted within this
                                   time of its arrival (0: never) */
+    uint64_t max_batch_butterflies; /* bntt_pm_run_trace closes a batch before a request that
+                                       would take its work, 3 * N/2 * log2 N butterflies per
+                                       request, above this (a batch holds at least one
+                                       request; 0: no limit beyond max_batch) */
 } bntt_pm_config;
 
 /* done_ns of a request that bntt_pm_run_trace dropped (its c is untouched). */

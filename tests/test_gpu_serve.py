@@ -228,6 +228,32 @@ def test_bench_serving_trace_sample(gpu, orc):
     del A, B, Cc, reqs
 
 
+def test_batch_work_cap(gpu):
+    """max_batch_butterflies closes a trace batch before a request that would
+    take its work above the cap: 40 requests of 2^10..2^13 arriving at once
+    under a cap of two 2^13-point requests (3 * 2^12 * 13 butterflies each)
+    and a batch limit of 64 take at least as many batches as the work
+    requires, and a cap below any one request gives one batch per request;
+    every result equals the same request alone."""
+    q = C.Q_116
+    one13 = 3 * (1 << 12) * 13
+    logs, hosts, dev, arrival = _trace(gpu, 40, q, 21, 10, 13, 0)
+    work = sum(3 * (1 << (int(L) - 1)) * int(L) for L in logs)
+    for cap, min_batches in ((2 * one13, -(-work // (2 * one13))), (1, 40)):
+        srv = server(q, hbm_budget=1 << 30, max_inflight=3, max_batch=64, min_log2n=10, max_log2n=13,
+                     max_batch_butterflies=cap)
+        outs = [gpu.empty_like(a) for a, _ in dev]
+        done, st = srv.run_trace([(a, b, c) for (a, b), c in zip(dev, outs)], arrival)
+        assert st["batches"] >= min_batches, (cap, st["batches"])
+        if cap == 1:
+            assert st["batches"] == 40
+        for i, ((a, b), c) in enumerate(zip(dev, outs)):
+            alone = gpu.empty_like(a)
+            srv.multiply([(a, b, alone)])
+            assert gpu.equal(alone, c), f"cap {cap} request {i}"
+        srv.close()
+
+
 def test_tight_budget_admission(gpu):
     """A budget of 2.5 requests of 2^12 points: arrived requests wait for
     budget, never more than two are on the device, and results are unchanged."""
