@@ -467,7 +467,7 @@ def serving(torch, np, n_req: int = 16384) -> dict:
     """Negacyclic polynomial-multiply requests mod q_116 (forward NTT x2,
     pointwise product, inverse NTT) with N uniform on 2^10..2^16, served with
     continuous batching under a 2 GiB HBM budget on 3 streams, batches closed
-    at 64 requests or 24 M butterflies.  First a closed
+    at 64 requests or 20 M butterflies.  First a closed
     loop (every request arrived at t = 0) gives the saturation rate; then
     seeded Poisson arrivals at fractions of it give p50/p99 latency (arrival to
     completion seen by the host) and the achieved rate.  16384 requests per
@@ -490,9 +490,9 @@ def serving(torch, np, n_req: int = 16384) -> dict:
     torch.cuda.synchronize()
     budget = 2 << 30
     streams = 3
-    # batches close at 64 requests or 24 M butterflies of work, whichever
+    # batches close at 64 requests or 20 M butterflies of work, whichever
     # comes first (a batch's duration follows its work)
-    cap = 24_000_000
+    cap = 20_000_000
     srv = PolymulServer(q, hbm_budget=budget, max_inflight=streams, max_batch=64, min_log2n=10, max_log2n=16,
                         max_batch_butterflies=cap)
     srv.multiply(reqs[:64])  # warm-up
