@@ -160,9 +160,10 @@ struct LoneShape {
 
 // Workgroups of a lone transform of 2^L points given a tmp (1: one workgroup).
 __host__ __device__ constexpr int lone_parts(int L) { return L == 11 ? 4 : L == 12 ? 8 : 1; }
-// Grid-barrier polls before a workgroup gives up on its siblings (about a
-// second; every workgroup of this library's kernels is resident far sooner).
-constexpr uint32_t kLoneSpinLimit = 1u << 24;
+// Grid-barrier polls before a workgroup gives up on its siblings: each poll is
+// a device-scope load of about 0.5-1 us, so about one to two seconds (every
+// workgroup of this library's kernels is resident far sooner).
+constexpr uint32_t kLoneSpinLimit = 1u << 21;
 
 // One stage of a group: p = tw * b, (a, b) <- (a + p, a - p) as the
 // reference's ntt_scalar.c:611-618.  FAST: digit twiddle, fast exact product.
