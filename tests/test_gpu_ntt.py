@@ -210,11 +210,37 @@ def test_inputs_just_below_q(gpu, orc):
 
 
 def test_other_modulus_literal_path(gpu, orc):
+    """A modulus outside the fast form runs the literal formulas: one-pass
+    and two-pass lone transforms (2^6, 2^10), 2^12 on one workgroup per
+    transform and a multi-pass size, two transforms each."""
     q = C.Q_116
     mu = (1 << 235) // q
-    for L in (6, 13):
+    for L in (6, 10, 12, 13):
         x, tw = uniform(2 << L, q, L), uniform(1 << L, q, L + 1)
         assert np.array_equal(ntt_dev(gpu, x, tw, q, mu), orc.ntt(x, tw, q, mu))
+
+
+@pytest.mark.parametrize("L", [8, 10])
+def test_lone_pass2_literal(gpu, orc, L):
+    """A table entry >= 2^124 that only the second pass reads (entries from
+    2^(L/2) on belong to its stages) sends that pass of the wavefronts using it
+    to the literal formulas while the first pass stays fast; three transforms
+    in one launch against the oracle."""
+    n = 1 << L
+    x, tw = uniform(3 * n, QN, 800 + L), uniform(n, QN, 801 + L)
+    tw[n - 5, 1] |= np.uint64(1 << 61)
+    tw[(1 << (L // 2)) + 3, 1] |= np.uint64(1 << 63)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
+
+
+@pytest.mark.parametrize("L,count", [(8, 64), (8, 65), (10, 64), (10, 65)])
+def test_lone_count_boundary(gpu, orc, L, count):
+    """Up to 64 transforms of at most 2^10 points run the lone kernel (one
+    workgroup each); 65 take the packed one-launch kernel: both sides of the
+    boundary against the oracle."""
+    n = 1 << L
+    x, tw = uniform(count * n, QN, 820 + L + count), uniform(n, QN, 821 + L)
+    assert np.array_equal(ntt_dev(gpu, x, tw), orc.ntt(x, tw, QN, MUN))
 
 
 def test_linearity_2_20(gpu):
