@@ -317,6 +317,17 @@ __global__ __launch_bounds__((LoneShape<L, P>::T)) void ntt_lone_kernel(
     }
 }
 
+// Grid-barrier tokens: one counter for every split launch of the process,
+// from a random start, so that no launch's token equals a stale barrier word
+// that an earlier launch (of any size, in this process or an earlier one)
+// left in the same memory.
+static uint64_t next_barrier_token()
+{
+    static std::atomic<uint64_t> next{((uint64_t)std::random_device{}() << 32) ^
+                                      (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count()};
+    return next.fetch_add(1) + 1;
+}
+
 template <int L, int P, bool PM>
 static hipError_t launch_lone_l(U128 *y, const U128 *x, const U128 *tw, size_t count, const Mod &md, U128 *tmp,
                                 hipStream_t stream, uint32_t *done, uint32_t seq)
@@ -325,12 +336,9 @@ static hipError_t launch_lone_l(U128 *y, const U128 *x, const U128 *tw, size_t c
     uint64_t *bar = nullptr;
     uint64_t token = 0;
     if constexpr (P > 1) {
-        // tokens: a per-process random start, then one per launch
-        static std::atomic<uint64_t> next{((uint64_t)std::random_device{}() << 32) ^
-                                          (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count()};
         zg = tmp;
         bar = reinterpret_cast<uint64_t *>(tmp + ((size_t)1 << L));
-        token = next.fetch_add(1) + 1;
+        token = next_barrier_token();
         count = P;
     }
     hipLaunchKernelGGL((ntt_lone_kernel<L, P, PM>), dim3((unsigned)count), dim3(LoneShape<L, P>::T), 0, stream, y, x,
