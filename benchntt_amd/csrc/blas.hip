@@ -16,7 +16,12 @@
 // 512-thread blocks and 7.0 TB/s in 256-thread blocks (1024: 6.7 TB/s; two
 // or four elements per lane: 6.8-6.9 TB/s).  Alternating 64/128/256-thread
 // blocks on three boxes, 128 was 0.5-1% ahead of 256 for every op (64: 1%
-// behind), so all four use 128.
+// behind), so all four use 128.  Chunks are handed out so that each XCD
+// streams one contiguous eighth of the arrays (xcd_chunk): on MI355X that took
+// axpy / vvadd / vvmul at 2^24 from 122.1 / 120.2 / 124.5 us to 114.1 / 113.5 /
+// 114.1 us on one box, and from 115.8 / 116.7 / 115.7 to 113.9 / 113.3 / 115.1
+// us on another (same process, alternating; 128 threads still best, 64 equal,
+// 256 and 512 slower, two elements per lane 3% slower).
 #include "kernels.hpp"
 
 namespace bntt {
@@ -47,11 +52,24 @@ __device__ __forceinline__ void store(U128 *p, const U128 &a)
     *reinterpret_cast<v4u *>(p) = t;
 }
 
+// Workgroups go to the XCDs round-robin by index (workgroup b to XCD b mod 8).
+// Chunk order: block b takes chunk (b mod 8) * (nb8 / 8) + b / 8 of the first
+// nb8 = nblocks - nblocks mod 8 chunks, so each XCD streams one contiguous
+// eighth of the arrays; the last nblocks mod 8 blocks keep their own chunks.
+// Any mapping here is a permutation of the chunks, so only speed depends on
+// where the workgroups actually run.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ size_t xcd_chunk(size_t blk, size_t nblocks)
+{
+    const size_t nb8 = nblocks - nblocks % kXcds;
+    return blk < nb8 ? (blk % kXcds) * (nb8 / kXcds) + blk / kXcds : blk;
+}
+
 template <int OP, bool PM>
 __global__ __launch_bounds__(kBlasThreads) void blas_kernel(U128 *__restrict__ c, const U128 *__restrict__ a,
                                                             const U128 *__restrict__ b, U128 x, size_t n, Mod md)
 {
-    const size_t base = (size_t)blockIdx.x * (kBlasThreads * kPerLane) + threadIdx.x;
+    const size_t base = xcd_chunk(blockIdx.x, gridDim.x) * (kBlasThreads * kPerLane) + threadIdx.x;
     U128 av[kPerLane], bv[kPerLane];
 #pragma unroll
     for (int j = 0; j < kPerLane; j++) {
