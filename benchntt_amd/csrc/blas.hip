@@ -52,19 +52,6 @@ __device__ __forceinline__ void store(U128 *p, const U128 &a)
     *reinterpret_cast<v4u *>(p) = t;
 }
 
-// Workgroups go to the XCDs round-robin by index (workgroup b to XCD b mod 8).
-// Chunk order: block b takes chunk (b mod 8) * (nb8 / 8) + b / 8 of the first
-// nb8 = nblocks - nblocks mod 8 chunks, so each XCD streams one contiguous
-// eighth of the arrays; the last nblocks mod 8 blocks keep their own chunks.
-// Any mapping here is a permutation of the chunks, so only speed depends on
-// where the workgroups actually run.
-constexpr unsigned kXcds = 8;
-__device__ __forceinline__ size_t xcd_chunk(size_t blk, size_t nblocks)
-{
-    const size_t nb8 = nblocks - nblocks % kXcds;
-    return blk < nb8 ? (blk % kXcds) * (nb8 / kXcds) + blk / kXcds : blk;
-}
-
 template <int OP, bool PM>
 __global__ __launch_bounds__(kBlasThreads) void blas_kernel(U128 *__restrict__ c, const U128 *__restrict__ a,
                                                             const U128 *__restrict__ b, U128 x, size_t n, Mod md)

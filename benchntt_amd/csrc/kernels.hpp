@@ -11,6 +11,20 @@ namespace bntt {
 
 enum BlasOp : int { OP_VVADD = 0, OP_VVSUB = 1, OP_VVMUL = 2, OP_AXPY = 3 };
 
+// Workgroups go to the XCDs round-robin by index (workgroup b to XCD b mod 8).
+// Chunk order of a grid of one-chunk workgroups: block b takes chunk
+// (b mod 8) * (nb8 / 8) + b / 8 of the first nb8 = nblocks - nblocks mod 8
+// chunks, so each XCD works through one contiguous eighth of the data; the
+// last nblocks mod 8 blocks keep their own chunks.  Any such order is a
+// permutation of the chunks, so only speed depends on where the workgroups
+// actually run.
+constexpr unsigned kXcds = 8;
+__device__ __forceinline__ size_t xcd_chunk(size_t blk, size_t nblocks)
+{
+    const size_t nb8 = nblocks - nblocks % kXcds;
+    return blk < nb8 ? (blk % kXcds) * (nb8 / kXcds) + blk / kXcds : blk;
+}
+
 // Elementwise ops over n residues (device pointers).  x is the axpy scalar.
 hipError_t launch_blas(int op, U128 *c, const U128 *a, const U128 *b, U128 x, size_t n, const Mod &md,
                        hipStream_t stream);

@@ -13,7 +13,9 @@ namespace bntt {
 
 // One pass: stages [S, S+k) of `count` transforms of size 2^L.
 // Group id gamma = transform * 2^(L-k) + M; this workgroup owns groups
-// [blockIdx.x * G, blockIdx.x * G + G).  Block size = (G << k) >> ELOG.
+// [c * G, c * G + G) for its chunk c = xcd_chunk(blockIdx.x) (each XCD takes
+// a contiguous range: 8192 x 1024 1.5% faster, one 2^20 / 2^24 transform 7% /
+// 3.5%, measured alternately on MI355X).  Block size = (G << k) >> ELOG.
 template <int ELOG, bool PM>
 __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict__ y, const U128 *__restrict__ x,
                                                                const U128 *__restrict__ tw, int L, int S, int k,
@@ -25,7 +27,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     const int T = blockDim.x;
     const int G = 1 << Glog;
     const int mlog = L - k;  // groups per transform = 2^mlog
-    const size_t gamma0 = (size_t)blockIdx.x << Glog;
+    const size_t gamma0 = xcd_chunk(blockIdx.x, gridDim.x) << Glog;
     const int nvalid = (int)((total_groups - gamma0) < (size_t)G ? (total_groups - gamma0) : (size_t)G);
     bool big = false;
 
