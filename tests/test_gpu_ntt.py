@@ -359,3 +359,35 @@ def test_lone_transform_split(gpu, orc, L):
     big[n + 8:2 * n + 8] = dx
     with pytest.raises(bn.BnttError):
         run(gpu.empty_like(dx), big[n + 8:2 * n + 8], dtw, big[:need])
+
+
+def test_lone_split_concurrent_streams(gpu, orc):
+    """Lone 2^12 transforms split over workgroups, enqueued on three streams
+    at once, each stream with its own tmp (own barrier words and tokens):
+    every stream's results match the oracle, over several rounds."""
+    import ctypes
+
+    from benchntt_amd import _lib
+
+    L, n = 12, 1 << 12
+    lib = _lib.load()
+    need = lib.bntt_ntt_tmp_bytes(L, 1) // 16
+    m, mu = _lib.U128.of(QN), _lib.U128.of(MUN)
+    streams = [gpu.cuda.Stream() for _ in range(3)]
+    tmps = [gpu.empty((need, 2), dtype=gpu.int64, device="cuda") for _ in streams]
+    tw = uniform(n, QN, 950)
+    dtw = to_dev(gpu, tw)
+    xs = [uniform(n, QN, 951 + i) for i in range(3)]
+    dxs = [to_dev(gpu, x) for x in xs]
+    wants = [orc.ntt(x, tw, QN, MUN) for x in xs]
+    gpu.cuda.synchronize()
+    for rnd in range(4):
+        ys = [gpu.empty_like(dx) for dx in dxs]
+        for _ in range(5):
+            for s, t, dx, dy in zip(streams, tmps, dxs, ys):
+                _lib.check(lib.bntt_ntt_fwd_dev(dy.data_ptr(), dx.data_ptr(), dtw.data_ptr(), L, 1, ctypes.byref(m),
+                                                ctypes.byref(mu), t.data_ptr(), ctypes.c_void_p(s.cuda_stream)))
+        gpu.cuda.synchronize()
+        for i, (dy, want) in enumerate(zip(ys, wants)):
+            assert np.array_equal(from_dev(dy), want), (rnd, i)
+
