@@ -126,14 +126,14 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
 // barrier (LDS executes a wavefront's accesses in order; `lds_order` keeps the
 // compiler from moving them).  Between the passes every value crosses once,
 // stored in the second pass's group order: through LDS when one workgroup
-// holds the transform (P = 1, up to 2^10 points, any number of transforms,
-// one workgroup each), or through device memory and a grid barrier when a
-// lone transform of 2^11 or 2^12 points is split over P = 4 or 8 workgroups
-// of 256 threads.  (At 2^10 points two workgroups measured slower than one:
-// 8.1 against 7.3 us per call, the barrier and the round trip through device
-// memory costing more than the halved network saves.)  Every input and the one twiddle of each
-// of the thread's L stages are loaded before the first butterfly, and all
-// loads precede every store (y may equal x).
+// holds the transform (P = 1, up to 2^10 points, one workgroup per transform
+// of the launch), or through device memory and a grid barrier when a lone
+// transform of 2^11 or 2^12 points is split over P = 4 or 8 workgroups of
+// 256 threads.  (At 2^10 points two workgroups measured slower than one: 8.1
+// against 7.3 us per call, the barrier and the round trip through device
+// memory costing more than the halved network saves.)  Every input and the
+// one twiddle of each of the thread's L stages are loaded before the first
+// butterfly, and all loads precede every store (y may equal x).
 //
 // Thread mapping: in pass 1 the groups of a wavefront interleave (group g of
 // the wavefront is lane g + GPW1 * i), so consecutive lanes load consecutive
