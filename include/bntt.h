@@ -102,10 +102,12 @@ size_t bntt_ntt_tmp_bytes(int log2n, size_t count);
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): bntt_ntt_fwd_dev without the tmp argument, allocating
- * nothing.  Multi-pass sizes (log2n > 12) keep their intermediates in a fixed
- * array of BNTT_NOALLOC_MAX_RESIDUES residues (16 MiB) that the library holds
- * on each GPU it runs on until the process ends, so count * 2^log2n may not
- * exceed it there (BNTT_EINVAL); one-pass sizes have no limit.  Every call on
+ * nothing.  Multi-pass sizes (log2n > 12), and one transform of 2^11 or 2^12
+ * points (split over several workgroups, as with a tmp), keep their
+ * intermediates in a fixed array of BNTT_NOALLOC_MAX_RESIDUES residues
+ * (16 MiB) that the library holds on each GPU it runs on until the process
+ * ends, so count * 2^log2n may not exceed it there (BNTT_EINVAL); one-pass
+ * sizes have no limit.  Every call on
  * one GPU shares that GPU's array: calls that overlap on the device (on
  * different streams, or from several threads) race and must be ordered by
  * the caller.  Results are those of bntt_ntt_fwd_dev (same kernels). */
