@@ -112,7 +112,7 @@ def _convert(name, src, out, stream):
     src = np.ascontiguousarray(src, dtype=np.uint64).reshape(-1, 2)
     if out is None:
         out = np.empty_like(src)
-    _lib.call(name, _host_ptr(out), _host_ptr(src), n)
+    _lib.call(name, _host_ptr(out, out=True), _host_ptr(src), n)
     return out
 
 

@@ -43,12 +43,15 @@ def _dev_ptr(a) -> int:
     return a.data_ptr()
 
 
-def _host_ptr(a: np.ndarray) -> int:
+def _host_ptr(a: np.ndarray, out: bool = False) -> int:
+    """Address of a host array's data; out: the library writes it."""
     if not isinstance(a, np.ndarray):
         raise ValueError("host arrays must be C-contiguous numpy arrays")
     fl = a.flags
     if not fl.c_contiguous:
         raise ValueError("host arrays must be C-contiguous numpy arrays")
+    if out and not fl.writeable:
+        raise ValueError("output arrays must be writeable")
     if fl.writeable and a.nbytes:
         # the buffer protocol gives the address faster than a.ctypes
         return ctypes.addressof(_c_char.from_buffer(a))
@@ -122,7 +125,7 @@ def _blas(name, c, a, b, extra_before_n, m, mu, stream):
         ptrs = [_dev_ptr(c), _dev_ptr(a), _dev_ptr(b)]
         _lib.call(f"bntt_{name}_dev", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu, _stream(stream, c))
     else:
-        ptrs = [_host_ptr(c), _host_ptr(a), _host_ptr(b)]
+        ptrs = [_host_ptr(c, out=True), _host_ptr(a), _host_ptr(b)]
         _lib.call(f"bntt_{name}", *ptrs, *extra_before_n, n, ctypes.byref(mm), *args_mu)
 
 
@@ -183,7 +186,7 @@ def _ntt_host(Y, X, tw, modulus, mu) -> None:
     if (total >> 4) % n:
         raise ValueError("X must hold a whole number of transforms")
     count = (total >> 4) // n
-    py, px, pt = _host_ptr(Y), _host_ptr(X), _host_ptr(tw)
+    py, px, pt = _host_ptr(Y, out=True), _host_ptr(X), _host_ptr(tw)
     if _fwd_host is None:
         _fwd_host = _lib.load().bntt_ntt_fwd
     mm, mmu = ctypes.byref(_u128(modulus)), ctypes.byref(_u128(mu))
@@ -243,7 +246,7 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
 
             own.record_stream(torch.cuda.ExternalStream(s.value or 0, device=X.device))
     else:
-        py, px, pt = _host_ptr(Y), _host_ptr(X), _host_ptr(twiddles)
+        py, px, pt = _host_ptr(Y, out=True), _host_ptr(X), _host_ptr(twiddles)
         for i in range(count):
             off = i * n * 16
             _lib.call("bntt_ntt_fwd", py + off, px + off, pt, log2n, ctypes.byref(mm), ctypes.byref(mmu))

@@ -114,3 +114,20 @@ def test_compat_header_compiles(tmp_path):
     subprocess.run(["gcc", "-std=c11", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
                     "-L", lib_dir, "-lbntt", "-Wl,-rpath," + lib_dir, "-o", str(tmp_path / "caller")],
                    check=True)
+
+
+def test_read_only_outputs_rejected():
+    """Host-array calls check their arguments before any library call: a
+    read-only output array is an argument error (no GPU needed)."""
+    import numpy as np
+
+    import benchntt_amd as bn
+    from benchntt_amd import config as C
+
+    x = np.zeros((16, 2), dtype=np.uint64)
+    y = np.zeros_like(x)
+    y.flags.writeable = False
+    with pytest.raises(ValueError, match="writeable"):
+        bn.ntt(y, x, C.Q_NTT, C.MU_NTT, x.copy())
+    with pytest.raises(ValueError, match="writeable"):
+        bn.vector_add(y, x, x, C.Q_BLAS)
