@@ -169,13 +169,15 @@ def ntt_sweep(torch, np, qn, mun, cpu: bool = True) -> dict:
     the last 50 of 100 calls): the plain one-transform GPU path on device
     arrays (bntt_ntt_fwd_dev, count 1, back-to-back calls timed with HIP
     events, and the same calls replayed from a captured graph, which leaves
-    out the host's cost of each call) beside the reference's scalar, AVX2 and AVX-512 ntt() on one host
-    core (its AMD build where the host has the ISA).  Large N on the CPU take
-    fewer calls (still the mean of the last half), so the sweep stays within
-    a few seconds.  Seeded uniform inputs and a seeded table per N."""
+    out the host's cost of each call), the drop-in on host arrays in the
+    reference's protocol (bntt_ntt_fwd timed per call in C, PCIe and
+    synchronisation included), and the reference's scalar, AVX2 and AVX-512
+    ntt() on one host core (its AMD build where the host has the ISA).  Large
+    N take fewer calls (still the mean of the last half), so the sweep stays
+    within a few seconds.  Seeded uniform inputs and a seeded table per N."""
     import ctypes
 
-    from benchntt_amd import _lib
+    from benchntt_amd import _lib, calltime
     from benchntt_amd._lib import U128
     from benchntt_amd.residues import uniform
 
@@ -229,8 +231,17 @@ def ntt_sweep(torch, np, qn, mun, cpu: bool = True) -> dict:
         torch.cuda.synchronize()
         gus = ev0.elapsed_time(ev1) / 100 * 1e3
         del g
+        # the reference's call shape and protocol: host arrays in and out, each
+        # call timed in C, mean of the last half; its output must equal the
+        # device path's
+        yh = np.empty_like(x_np)
+        hns = calltime.ntt_ns_per_call(yh, x_np, tw_np, qn, mun, iters=calls)
+        if not np.array_equal(yh, y.cpu().numpy().view(np.uint64).reshape(yh.shape)):
+            raise RuntimeError(f"host-array ntt() at N=2^{L} differs from the device path")
         row = {"log2n": L, "gpu_plain_us": round(us, 2), "gpu_plain_ns_per_butterfly": round(us * 1e3 / bfly, 4),
-               "gpu_plain_graph_us": round(gus, 2), "gpu_plain_graph_ns_per_butterfly": round(gus * 1e3 / bfly, 4)}
+               "gpu_plain_graph_us": round(gus, 2), "gpu_plain_graph_ns_per_butterfly": round(gus * 1e3 / bfly, 4),
+               "gpu_host_entry_us": round(hns / 1e3, 2), "gpu_host_entry_ns_per_butterfly": round(hns / bfly, 4),
+               "gpu_host_entry_calls": calls}
         if have_ref:
             for isa in isas:
                 _, ns1 = ref.ntt(x_np, tw_np, qn, mun, isa=isa, iters=2, fl=fl)
@@ -241,7 +252,8 @@ def ntt_sweep(torch, np, qn, mun, cpu: bool = True) -> dict:
                 row[f"cpu_{isa}_calls"] = iters
         rows.append(row)
         log(f"sweep N=2^{L}: GPU plain {us:.2f} us ({row['gpu_plain_ns_per_butterfly']} ns/butterfly), "
-            f"{gus:.2f} us from a graph"
+            f"{gus:.2f} us from a graph, host arrays {hns / 1e3:.2f} us ({row['gpu_host_entry_ns_per_butterfly']} "
+            "ns/butterfly)"
             + "".join(f", {k.split('_')[1]} {v} ns/butterfly" for k, v in row.items()
                       if k.startswith("cpu_") and k.endswith("butterfly")))
     out = {"ntt_sweep": rows}
@@ -257,8 +269,10 @@ def blas_1024_rows(torch, np, bn, q, mu, s, cpu: bool = True) -> dict:
     100 calls; the AVX-512 add and sub take the last 500 of 1000): each op on
     the GPU through device arrays (back-to-back launches, HIP events) and
     through the reference's call shape (host arrays in and out, one call at a
-    time, from Python), beside the reference's scalar, AVX2 and AVX-512 kernels
-    on one host core.  Operands: seeded uniform residues mod q_B."""
+    time, timed per call in C as the reference does, and from Python), beside
+    the reference's scalar, AVX2 and AVX-512 kernels on one host core.
+    Operands: seeded uniform residues mod q_B."""
+    from benchntt_amd import calltime
     from benchntt_amd.residues import uniform
 
     n = 1024
@@ -287,12 +301,20 @@ def blas_1024_rows(torch, np, bn, q, mu, s, cpu: bool = True) -> dict:
     for op, (fdev, fhost) in fns.items():
         ms = time_kernel(torch, fdev, 50, 50, st)
         row = {"gpu_device_ns_per_element": round(ms * 1e6 / n, 3)}
+        # host arrays, timed per call in C in the reference's protocol; the
+        # output must equal the device path's
+        ns = calltime.blas_ns_per_call(op, ch, a_np, b_np, q, mu, s, iters=100)
+        row["gpu_host_entry_ns_per_element"] = round(ns / n, 3)
+        fdev()
+        torch.cuda.synchronize()
+        if not np.array_equal(ch, dc.cpu().numpy().view(np.uint64).reshape(ch.shape)):
+            raise RuntimeError(f"host-array {op} differs from the device path")
         for _ in range(20):
             fhost()
         t0 = time.perf_counter()
         for _ in range(100):
             fhost()
-        row["gpu_host_entry_ns_per_element"] = round((time.perf_counter() - t0) / 100 * 1e9 / n, 3)
+        row["gpu_host_entry_python_ns_per_element"] = round((time.perf_counter() - t0) / 100 * 1e9 / n, 3)
         for isa, ops in ref_ops.items():
             iters = 1000 if (isa == "avx512" and op in ("vvadd", "vvsub")) else 100
             row[f"cpu_{isa}_ns_per_element"] = round(ref.blas_ns_per_element(op, ops, q, mu, s, iters), 3)
@@ -306,6 +328,7 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     and N in {16..512} with 2^23 coefficients, one shared seeded twiddle table
     per N, as one persistent work-queue execution (graph replay) against one
     launch per transform from native code."""
+    from benchntt_amd import calltime
     from benchntt_amd.batch import BatchPlan, TaskList, launch_each, load_tune_table
     from benchntt_amd.residues import uniform
 
@@ -320,16 +343,22 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     out["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
     out["ntt_plain_1024_butterflies_per_s"] = round(5120 / (ms1 * 1e-3), 1)
     # the reference's call shape: host arrays in, host array out, one call at a
-    # time (PCIe and synchronisation included; the library's zero-copy path)
+    # time (PCIe and synchronisation included; the library's zero-copy path),
+    # timed per call in C as the reference times ntt() (mean of the last 100
+    # of 200 calls), and the same calls from Python (bn.ntt on numpy arrays)
     yh = np.empty_like(x_np)
+    out["ntt_plain_1024_host_entry_us"] = round(calltime.ntt_ns_per_call(yh, x_np, tw_np, qn, mun, 200) / 1e3, 2)
+    if not np.array_equal(yh, y1.cpu().numpy().view(np.uint64).reshape(yh.shape)):
+        raise RuntimeError("host-array ntt() at N=1024 differs from the device path")
     for _ in range(20):
         bn.ntt(yh, x_np, qn, mun, tw_np)
     t0 = time.perf_counter()
     for _ in range(200):
         bn.ntt(yh, x_np, qn, mun, tw_np)
-    out["ntt_plain_1024_host_entry_us"] = round((time.perf_counter() - t0) / 200 * 1e6, 2)
+    out["ntt_plain_1024_host_entry_python_us"] = round((time.perf_counter() - t0) / 200 * 1e6, 2)
     log(f"ntt plain 1024: {ms1 * 1e3:.2f} us per launch on device arrays, "
-        f"{out['ntt_plain_1024_host_entry_us']} us per host-array call")
+        f"{out['ntt_plain_1024_host_entry_us']} us per host-array call timed in C, "
+        f"{out['ntt_plain_1024_host_entry_python_us']} us from Python")
     tune = load_tune_table()
     total = 1 << 23
     xb = torch.from_numpy(uniform(total, qn, 5).view(np.int64)).to(dev)

@@ -1,4 +1,6 @@
-"""Build libbntt.so (the HIP kernels and the C ABI of include/bntt.h) in-tree.
+"""Build libbntt.so (the HIP kernels and the C ABI of include/bntt.h) in-tree,
+and libbntt_calltime.so beside it (the reference's timing protocol in C over
+the host-array entry points, used by bench.py).
 
     python benchntt_amd/build.py
 
@@ -20,6 +22,8 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbntt.so")
 SOURCES = ["api.hip", "blas.hip", "ntt.hip", "batch.hip", "serve.hip", "io.hip", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CC = os.environ.get("CC", "gcc")
+CALLTIME_LIB = os.path.join(HERE, "libbntt_calltime.so")
 
 CFLAGS = [
     "-O3",
@@ -54,7 +58,23 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_calltime(verbose)
     return LIB
+
+
+def build_calltime(verbose: bool = False) -> str:
+    """libbntt_calltime.so: the reference's per-call timing protocol in C over
+    the host-array entry points (csrc/call_timing.c), linked to libbntt.so."""
+    tmp = CALLTIME_LIB + ".tmp"
+    cmd = [CC, "-O2", "-std=gnu11", "-Wall", "-Werror", "-fPIC", "-shared", "-I", os.path.join(ROOT, "include"),
+           os.path.join(CSRC, "call_timing.c"), "-L", HERE, "-lbntt", "-Wl,-rpath,$ORIGIN", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{CC} failed on call_timing.c:\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, CALLTIME_LIB)
+    if verbose:
+        print(f"built {CALLTIME_LIB}")
+    return CALLTIME_LIB
 
 
 if __name__ == "__main__":

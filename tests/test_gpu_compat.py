@@ -99,3 +99,27 @@ def test_reference_style_caller(gpu, fixtures, tmp_path, stages):
     if stages == 10:
         for op in ("vvadd", "vvsub", "vvmul", "axpy", "mulmod128"):
             assert f"{op} ok" in lines, r.stdout
+
+
+@pytest.mark.parametrize("n", [16, 1024])
+def test_calltime_ntt_output(gpu, fixtures, n):
+    """The reference-protocol timing harness (calls timed in C) leaves the
+    golden output of its last host-array ntt() call in Y."""
+    from benchntt_amd import calltime
+
+    x, tw = fixtures[f"x_{n}"].copy(), fixtures[f"twd_{n}"].copy()
+    y = np.zeros_like(x)
+    ns = calltime.ntt_ns_per_call(y, x, tw, C.Q_NTT, C.MU_NTT, iters=6)
+    assert ns > 0
+    assert np.array_equal(y, fixtures[f"ver_{n}"])
+
+
+@pytest.mark.parametrize("op", ["vvadd", "vvsub", "vvmul", "axpy"])
+def test_calltime_blas_output(gpu, fixtures, op):
+    from benchntt_amd import calltime
+
+    a, b = fixtures["twd_1024"].copy(), fixtures["x_1024"].copy()
+    c = np.zeros_like(a)
+    ns = calltime.blas_ns_per_call(op, c, a, b, C.Q_BLAS, C.MU_BLAS, C.AXPY_SCALAR, iters=6)
+    assert ns > 0
+    assert np.array_equal(c, fixtures[f"{op}_1024"])

@@ -66,6 +66,33 @@ def test_no_cpu_fallback_without_gpu():
         bn.ntt(a, a, bn.config.Q_NTT, bn.config.MU_NTT, a)
 
 
+def test_calltime_library_loads():
+    """The timing harness (the reference's per-call protocol in C) links to
+    libbntt.so and exports its two entry points; without a GPU a timed call
+    reports the library's error instead of a time."""
+    import torch
+
+    from benchntt_amd import calltime
+
+    out = subprocess.run(["nm", "-D", "--defined-only", calltime.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    assert exported == {"bntt_calltime_ntt", "bntt_calltime_blas"}
+    calltime.load()
+    with pytest.raises(ValueError):
+        calltime.ntt_ns_per_call(np.zeros((16, 2), np.uint64), np.zeros((8, 2), np.uint64),
+                                 np.zeros((16, 2), np.uint64), 17, 1)
+    if torch.cuda.is_available():
+        return
+    import benchntt_amd as bn
+
+    a = np.zeros((16, 2), np.uint64)
+    with pytest.raises(bn.BnttError):
+        calltime.ntt_ns_per_call(a.copy(), a, a, bn.config.Q_NTT, bn.config.MU_NTT, iters=4)
+    with pytest.raises(bn.BnttError):
+        calltime.blas_ns_per_call("axpy", a.copy(), a, a, bn.config.Q_BLAS, bn.config.MU_BLAS, 3, iters=4)
+
+
 def test_struct_layouts_match_header(tmp_path):
     """Every ctypes structure of the Python binding has the size and field
     offsets of its C struct in include/bntt.h (compiled with gcc)."""
