@@ -412,12 +412,12 @@ size_t bntt_ntt_tmp_bytes(int log2n, size_t count)
 {
     if (log2n < bntt::kMinLog2 || log2n > bntt::kMaxLog2) return 0;
     // one pass on one workgroup per transform, except a lone transform of
-    // 2^11 or 2^12 points, which with a tmp (its intermediate and 16 residues
-    // of grid-barrier words) splits over several workgroups
-    if (log2n <= bntt::kLdsMaxLog2) {
-        if (count != 1 || log2n < bntt::kLoneSplitMinLog2) return 0;
+    // 2^11 .. 2^13 points, which with a tmp (its intermediate and 16 residues
+    // of grid-barrier words) splits over several workgroups; above 4096 points
+    // the tmp is required (the intermediate of several passes)
+    if (count == 1 && log2n >= bntt::kLoneSplitMinLog2 && log2n <= bntt::kLoneSplitMaxLog2)
         return (((size_t)1 << log2n) + 16) * sizeof(bntt_u128);
-    }
+    if (log2n <= bntt::kLdsMaxLog2) return 0;
     if (count > (SIZE_MAX / sizeof(bntt_u128)) >> log2n) return SIZE_MAX;  // no such buffer exists
     return (count << log2n) * sizeof(bntt_u128);
 }
@@ -471,7 +471,7 @@ int bntt_ntt_fwd_dev_noalloc(void *y, const void *x, const void *tw, int log2n, 
         return fail(BNTT_EINVAL, "log2n = %d outside [%d, %d]", log2n, bntt::kMinLog2, bntt::kMaxLog2);
     void *tmp = nullptr;
     if (count > 0 && bntt_ntt_tmp_bytes(log2n, count) != 0) {
-        // multi-pass sizes, and a lone transform of 2^11 or 2^12 points split
+        // multi-pass sizes, and a lone transform of 2^11 .. 2^13 points split
         // over several workgroups, keep their intermediates in the fixed
         // scratch array; every element of it that a later pass reads is written
         // by an earlier pass of the same call, and the split's barrier words

@@ -32,10 +32,10 @@ hipError_t launch_blas(int op, U128 *c, const U128 *a, const U128 *b, U128 x, si
 // Forward constant-geometry NTT of `count` transforms of size 2^log2n, stored
 // contiguously, all with twiddle table tw (2^log2n entries).  tmp must hold
 // count * 2^log2n residues when log2n > kLdsMaxLog2 (multi-pass sizes); it may
-// be null otherwise.  Given for one transform of 2^11 or 2^12 points, it must
+// be null otherwise.  Given for one transform of 2^11 .. 2^13 points, it must
 // hold 2^log2n + 16 residues (the intermediate and the grid-barrier words),
 // and the transform splits over several workgroups (or, in a launch being
-// captured into a graph, takes two launches).  x and y may
+// captured into a graph, takes one workgroup or two launches).  x and y may
 // alias only when log2n <= kLdsMaxLog2.  With `done` (one transform of at
 // most 2^kLdsMaxLog2 points) each of the launch's *done_words workgroups w
 // stores `seq` to done[w], at system scope, after its outputs.
@@ -45,6 +45,7 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int log2n,
 
 constexpr int kLdsMaxLog2 = 12;   // transforms up to 4096 points run in one LDS-resident pass
 constexpr int kLoneSplitMinLog2 = 11;  // a lone transform from 2^11 points given a tmp splits over workgroups
+constexpr int kLoneSplitMaxLog2 = 13;  // ... up to 2^13 points (beyond, the barrier over 32 workgroups costs more than it saves)
 constexpr int kSpreadMinLog2 = 11;  // ... except a lone one from 2^11 points given a tmp under capture: two launches
 constexpr int kMinLog2 = 1;
 constexpr int kMaxLog2 = 24;

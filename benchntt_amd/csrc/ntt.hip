@@ -1,7 +1,8 @@
 // ntt.hip -- launch-per-call forward NTT: `count` transforms of one size with
 // one twiddle table, in one launch (N <= 4096) or two or three launches
-// through device memory (N > 4096, and a lone transform of 2^11 or 2^12
-// points given a tmp).  The network and its blocking are described in
+// through device memory (N > 4096), or one launch over several workgroups
+// joined by a grid barrier (a lone transform of 2^11 .. 2^13 points given a
+// tmp).  The network and its blocking are described in
 // ntt_core.hpp.
 #include "internal.hpp"
 #include "ntt_core.hpp"
@@ -128,10 +129,13 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
 // stored in the second pass's group order: through LDS when one workgroup
 // holds the transform (P = 1, up to 2^10 points, one workgroup per transform
 // of the launch), or through device memory and a grid barrier when a lone
-// transform of 2^11 or 2^12 points is split over P = 4 or 8 workgroups of
-// 256 threads.  (At 2^10 points two workgroups measured slower than one: 8.1
-// against 7.3 us per call, the barrier and the round trip through device
-// memory costing more than the halved network saves.)  Every input and the
+// transform of 2^11, 2^12 or 2^13 points is split over P = 4, 8 or 16
+// workgroups of 256 threads.  (At 2^10 points two workgroups measured slower
+// than one: 8.1 against 7.3 us per call, the barrier and the round trip
+// through device memory costing more than the halved network saves.  Half as
+// many workgroups of 512 threads were slower at 2^11 .. 2^14: 10.8 against
+// 8.6 us at 2^11, 11.8 against 10.1 us at 2^13; and 2^14 over 32 workgroups,
+// 13.2 us, did not beat its two launches, 13.0 us.)  Every input and the
 // one twiddle of each of the thread's L stages are loaded before the first
 // butterfly, and all loads precede every store (y may equal x).
 //
@@ -161,7 +165,10 @@ struct LoneShape {
 };
 
 // Workgroups of a lone transform of 2^L points given a tmp (1: one workgroup).
-__host__ __device__ constexpr int lone_parts(int L) { return L == 11 ? 4 : L == 12 ? 8 : 1; }
+__host__ __device__ constexpr int lone_parts(int L)
+{
+    return L == 11 ? 4 : L == 12 ? 8 : L == 13 ? 16 : 1;
+}
 // Grid-barrier polls before a workgroup gives up on its siblings: each poll is
 // a device-scope load of about 0.5-1 us, so about one to two seconds (every
 // workgroup of this library's kernels is resident far sooner).
@@ -356,6 +363,7 @@ static hipError_t launch_lone(U128 *y, const U128 *x, const U128 *tw, int L, siz
         switch (L) {
         case 11: return launch_lone_l<11, 4, PM>(y, x, tw, 1, md, tmp, stream, done, seq);
         case 12: return launch_lone_l<12, 8, PM>(y, x, tw, 1, md, tmp, stream, done, seq);
+        case 13: return launch_lone_l<13, 16, PM>(y, x, tw, 1, md, tmp, stream, done, seq);
         default: return hipErrorInvalidValue;
         }
     }
@@ -429,14 +437,15 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
     if (done_words) *done_words = 1;
     if (count == 0) return hipSuccess;
     if (L < kMinLog2 || L > kMaxLog2) return hipErrorInvalidValue;
-    // a completion flag needs one transform of at most 4096 points
-    if (done && (count != 1 || L > kLdsMaxLog2)) return hipErrorInvalidValue;
+    // a completion flag needs one transform of at most 2^13 points, and above
+    // 4096 points the split launch
+    if (done && (count != 1 || L > kLoneSplitMaxLog2)) return hipErrorInvalidValue;
     const bool pm = md.pm != 0;
-    // A lone transform of 2^11 or 2^12 points given a tmp splits over several
+    // A lone transform of 2^11 .. 2^13 points given a tmp splits over several
     // workgroups joined by a grid barrier.  Its per-launch token would repeat
     // in every replay of a captured graph, so a launch being captured takes
     // the paths below instead.
-    if (tmp != nullptr && count == 1 && L >= kLoneSplitMinLog2 && L <= kLdsMaxLog2) {
+    if (tmp != nullptr && count == 1 && L >= kLoneSplitMinLog2 && L <= kLoneSplitMaxLog2) {
         hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
         const hipError_t ce = hipStreamIsCapturing(stream, &cs);
         if (ce != hipSuccess) return ce;
@@ -446,6 +455,7 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
                       : launch_lone<false>(y, x, tw, L, 1, md, tmp, stream, done, seq);
         }
     }
+    if (done && L > kLdsMaxLog2) return hipErrorInvalidValue;
     if (L <= kLoneMaxLog2 && count <= kFewTransforms)
         return pm ? launch_lone<true>(y, x, tw, L, count, md, nullptr, stream, done, seq)
                   : launch_lone<false>(y, x, tw, L, count, md, nullptr, stream, done, seq);

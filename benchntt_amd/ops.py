@@ -155,8 +155,9 @@ _TMP_CACHE: dict = {}
 
 def tmp_elements(log2n: int, count: int = 1) -> int:
     """Residues of scratch the device NTT uses: 0 for log2n <= 12, except a
-    lone transform of 2^11 or 2^12 points (optional there, 2^log2n + 16
-    residues: with it the transform splits over several workgroups)."""
+    lone transform of 2^11 .. 2^13 points (2^log2n + 16 residues: with it the
+    transform splits over several workgroups; optional up to 2^12), and
+    count * 2^log2n above 2^12 points otherwise."""
     key = (log2n, count)
     v = _TMP_CACHE.get(key)
     if v is None:
@@ -204,10 +205,10 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
     `twiddles` (N entries).  N = len(twiddles); X and Y hold a whole number of
     transforms of N points (one, as in the reference, or a uniform batch).
     as0/as1 are the reference's scratch arguments: on the device path as0
-    serves as the scratch when the launch uses one (N > 4096: len >= len(X);
-    a lone transform of 2^11 or 2^12 points, which then splits over several
-    workgroups: len >= N + 16, else a scratch of that size is allocated); as1
-    is accepted and unused."""
+    serves as the scratch when the launch uses one (one transform of 2^11 ..
+    2^13 points, which then splits over several workgroups: len >= N + 16,
+    else a scratch of that size is allocated; several transforms of more than
+    4096 points: len >= len(X)); as1 is accepted and unused."""
     del as1
     if type(Y) is np.ndarray and type(X) is np.ndarray and type(twiddles) is np.ndarray:
         _ntt_host(Y, X, twiddles, modulus, mu)
@@ -227,9 +228,11 @@ def ntt(Y, X, modulus: int, mu: int, twiddles, as0=None, as1=None, stream=None) 
         own = None
         if need:
             if as0 is not None and _nelem(as0) < need:
-                if log2n > 12:
+                if count != 1 or log2n > 13:
                     raise ValueError(f"as0 must hold {need} residues for this N")
-                as0 = None  # optional scratch too small for the split (N + 16): use our own
+                # too small for the split of one transform (N + 16, where the
+                # reference's as0 holds N): use our own
+                as0 = None
             if as0 is None:
                 import torch
 

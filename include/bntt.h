@@ -87,22 +87,25 @@ int bntt_axpy_dev(void *c, const void *a, const void *b, const bntt_u128 *x, siz
  * y may equal x.  For log2n > 12 the transform takes several passes through
  * device memory: tmp must point to bntt_ntt_tmp_bytes(log2n, count) bytes of
  * device memory and y must not overlap x.  count = 1 is the plain
- * one-transform path; for log2n 11 and 12 with count = 1, tmp is optional
- * (bntt_ntt_tmp_bytes bytes, 2^log2n + 16 residues): given, the transform
- * splits over 4 or 8 workgroups joined by a grid barrier in one launch (in a
- * launch being captured into a graph: two launches).  tmp must not overlap x or y, and a call that
+ * one-transform path; for log2n 11 .. 13 with count = 1 the tmp holds
+ * bntt_ntt_tmp_bytes bytes, 2^log2n + 16 residues (optional at 11 and 12):
+ * given, the transform splits over 4, 8 or 16 workgroups joined by a grid
+ * barrier in one launch (in a launch being captured into a graph: one
+ * workgroup, or two launches).  tmp must not overlap x or y, and a call that
  * uses it must not run concurrently with another call using the same tmp. */
 int bntt_ntt_fwd_dev(void *y, const void *x, const void *tw, int log2n, size_t count, const bntt_u128 *m,
                      const bntt_u128 *mu, void *tmp, void *stream);
 /* Bytes of tmp for bntt_ntt_fwd_dev: 0 when log2n <= 12 (no tmp needed),
  * except count = 1 at log2n 11 and 12 (an optional tmp of 2^log2n + 16
- * residues), SIZE_MAX when count * 2^log2n * 16 does not fit a size_t. */
+ * residues); count * 2^log2n residues above that, except count = 1 at log2n
+ * 13 (2^log2n + 16 residues); SIZE_MAX when count * 2^log2n * 16 does not fit
+ * a size_t. */
 size_t bntt_ntt_tmp_bytes(int log2n, size_t count);
 
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): bntt_ntt_fwd_dev without the tmp argument, allocating
- * nothing.  Multi-pass sizes (log2n > 12), and one transform of 2^11 or 2^12
+ * nothing.  Multi-pass sizes (log2n > 12), and one transform of 2^11 .. 2^13
  * points (split over several workgroups, as with a tmp), keep their
  * intermediates in a fixed array of BNTT_NOALLOC_MAX_RESIDUES residues
  * (16 MiB) that the library holds on each GPU it runs on until the process

@@ -272,17 +272,18 @@ def test_bad_sizes(gpu):
         bn.ntt(x[:3], x[:3], QN, MUN, tw3)
 
 
-@pytest.mark.parametrize("L", [11, 12])
+@pytest.mark.parametrize("L", [11, 12, 13])
 def test_lone_transform_split(gpu, orc, L):
-    """A lone 2^11 or 2^12 transform given a tmp (2^L + 16 residues) splits
-    over 4 or 8 workgroups joined by a grid barrier: out of place,
-    in place and three calls in a row on one tmp (each launch's barrier token
-    differs from the stale words of the last) against the oracle, the same
-    result without a tmp (one workgroup), with an input above q and with a
-    pass-2 twiddle >= 2^124 (literal formulas on some wavefronts), over a
-    modulus outside the fast form, under graph capture (which takes the
-    unsplit paths) with replays on refilled inputs, and a tmp overlapping x
-    rejected."""
+    """A lone transform of 2^11 .. 2^13 points given a tmp (2^L + 16
+    residues) splits over 4, 8 or 16 workgroups joined by a grid barrier:
+    out of place and three calls in a row on one tmp (each launch's barrier
+    token differs from the stale words of the last) against the oracle; up to
+    2^12 also in place and the same result without a tmp (one workgroup),
+    above that both rejected (the tmp is required and y must not overlap x);
+    with an input above q and with a pass-2 twiddle >= 2^124 (literal formulas
+    on some wavefronts), over a modulus outside the fast form, under graph
+    capture (which takes the unsplit paths: one workgroup, or two passes) with
+    replays on refilled inputs, and a tmp overlapping x rejected."""
     import ctypes
 
     import benchntt_amd as bn
@@ -308,12 +309,18 @@ def test_lone_transform_split(gpu, orc, L):
         gpu.cuda.synchronize()
         assert np.array_equal(from_dev(dy), want), seed
     dz = dx.clone()
-    run(dz, dz, dtw, tmp)
     dw = gpu.empty_like(dx)
-    run(dw, dx, dtw, None)
-    gpu.cuda.synchronize()
-    assert np.array_equal(from_dev(dz), want)
-    assert np.array_equal(from_dev(dw), want)
+    if L <= 12:
+        run(dz, dz, dtw, tmp)
+        run(dw, dx, dtw, None)
+        gpu.cuda.synchronize()
+        assert np.array_equal(from_dev(dz), want)
+        assert np.array_equal(from_dev(dw), want)
+    else:
+        with pytest.raises(bn.BnttError):
+            run(dz, dz, dtw, tmp)
+        with pytest.raises(bn.BnttError):
+            run(dw, dx, dtw, None)
 
     # literal formulas: an input above q (pass 1 of one wavefront, then pass 2
     # of every wavefront that reads its outputs) and a table entry >= 2^124
@@ -361,15 +368,42 @@ def test_lone_transform_split(gpu, orc, L):
         run(gpu.empty_like(dx), big[n + 8:2 * n + 8], dtw, big[:need])
 
 
-def test_lone_split_concurrent_streams(gpu, orc):
-    """Lone 2^12 transforms split over workgroups, enqueued on three streams
-    at once, each stream with its own tmp (own barrier words and tokens):
-    every stream's results match the oracle, over several rounds."""
+@pytest.mark.parametrize("L", [12, 13])
+def test_python_as0_sizes(gpu, orc, L):
+    """bn.ntt's as0, the reference's scratch argument: one transform with an
+    as0 of the reference's N residues (too small for the split's N + 16: the
+    wrapper allocates its own) or of N + 16 (used), and several transforms of
+    2^13 points with an as0 shorter than count * N rejected."""
+    import benchntt_amd as bn
+
+    n = 1 << L
+    x, tw = uniform(n, QN, 960 + L), uniform(n, QN, 961 + L)
+    want = orc.ntt(x, tw, QN, MUN)
+    dx, dtw = to_dev(gpu, x), to_dev(gpu, tw)
+    for extra in (0, 16):
+        as0 = gpu.zeros((n + extra, 2), dtype=gpu.int64, device="cuda")
+        dy = gpu.empty_like(dx)
+        bn.ntt(dy, dx, QN, MUN, dtw, as0=as0)
+        gpu.cuda.synchronize()
+        assert np.array_equal(from_dev(dy), want), extra
+        assert bool((as0 != 0).any()) == (extra == 16), extra
+    if L > 12:
+        x2 = gpu.cat([dx, dx])
+        with pytest.raises(ValueError):
+            bn.ntt(gpu.empty_like(x2), x2, QN, MUN, dtw, as0=gpu.empty((n, 2), dtype=gpu.int64, device="cuda"))
+
+
+@pytest.mark.parametrize("L", [12, 13])
+def test_lone_split_concurrent_streams(gpu, orc, L):
+    """Lone 2^12 or 2^13 transforms split over 8 or 16 workgroups, enqueued
+    on three streams at once, each stream with its own tmp (own barrier words
+    and tokens): every stream's results match the oracle, over several
+    rounds."""
     import ctypes
 
     from benchntt_amd import _lib
 
-    L, n = 12, 1 << 12
+    n = 1 << L
     lib = _lib.load()
     need = lib.bntt_ntt_tmp_bytes(L, 1) // 16
     m, mu = _lib.U128.of(QN), _lib.U128.of(MUN)

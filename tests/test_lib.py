@@ -42,10 +42,13 @@ def test_loads_and_versions():
     assert lib.bntt_version() == 100
     assert lib.bntt_ntt_tmp_bytes(12, 5) == 0
     assert lib.bntt_ntt_tmp_bytes(10, 1) == 0
-    # optional: with it a lone 2^11 or 2^12 splits over several workgroups
-    # (intermediate plus 16 residues of grid-barrier words)
-    assert lib.bntt_ntt_tmp_bytes(11, 1) == ((1 << 11) + 16) * 16
-    assert lib.bntt_ntt_tmp_bytes(12, 1) == ((1 << 12) + 16) * 16
+    # optional: with it a lone 2^11 or 2^12 transform splits over workgroups
+    # (intermediate plus 16 residues of grid-barrier words); required from
+    # 2^13 on, where one transform of 2^13 points splits the same way
+    for L in (11, 12, 13):
+        assert lib.bntt_ntt_tmp_bytes(L, 1) == ((1 << L) + 16) * 16
+    assert lib.bntt_ntt_tmp_bytes(14, 1) == (1 << 14) * 16
+    assert lib.bntt_ntt_tmp_bytes(13, 2) == 2 * (1 << 13) * 16
     assert lib.bntt_ntt_tmp_bytes(11, 2) == 0
     assert lib.bntt_ntt_tmp_bytes(12, 2) == 0
     assert lib.bntt_ntt_tmp_bytes(16, 3) == 3 * (1 << 16) * 16
