@@ -234,6 +234,11 @@ int stage_for_current(Staging **out)
 
 // Largest arrays (bytes each) that the host-pointer entry points move zero-copy.
 constexpr size_t kZeroCopyBytes = (size_t)4096 * sizeof(bntt_u128);
+// Host-array transforms up to this size (2^16 points) are staged through
+// pinned memory, with DMA copies in and the result written straight back;
+// larger ones copy the caller's (pageable) arrays with the runtime's own
+// copies, which then run faster than one thread's copies into pinned memory.
+constexpr size_t kStagedBytes = ((size_t)1 << 16) * sizeof(bntt_u128);
 
 // The device arena, at least `bytes` (caller holds st.mu, current device = st's).
 int stage_reserve(Staging &stg, size_t bytes)
@@ -550,6 +555,36 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
     char *base = static_cast<char *>(stg.buf);
     void *dx = base, *dtw = base + bytes, *dy = base + 2 * bytes;
     void *dtmp = tmpb ? base + 3 * bytes : nullptr;
+    if (bytes <= kStagedBytes) {
+        // the caller's table is copied into pinned memory and DMA moves it to
+        // the device while x is copied into pinned memory; the transform reads
+        // x there (each element once) and the table in device memory, writes y
+        // straight into pinned memory, and y is copied out once the launch's
+        // completion words (one launch over several workgroups) or the stream
+        // (several passes) say it is there
+        if ((rc = stage_host_reserve(stg, 3 * bytes)) || (rc = stage_flag(stg))) return rc;
+        char *hb = static_cast<char *>(stg.hbuf), *hd = static_cast<char *>(stg.hdev);
+        std::memcpy(hb + bytes, tw, bytes);
+        hipError_t e = hipMemcpyAsync(dtw, hb + bytes, bytes, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");
+        std::memcpy(hb, X, bytes);
+        dx = hd;
+        U128 *yh = reinterpret_cast<U128 *>(hd + 2 * bytes);
+        if (log2n <= bntt::kLoneSplitMaxLog2) {
+            const uint32_t seq = ++stg.seq;
+            int words = 1;
+            e = bntt::launch_ntt_uniform(yh, static_cast<const U128 *>(dx), static_cast<const U128 *>(dtw), log2n, 1,
+                                         make_mod(m, mu), static_cast<U128 *>(dtmp), s, stg.flag_d, seq, &words);
+            if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
+            if ((rc = wait_flag(stg, seq, words))) return rc;
+        } else {
+            if ((rc = bntt_ntt_fwd_dev(yh, dx, dtw, log2n, 1, m, mu, dtmp, s))) return rc;
+            e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        }
+        std::memcpy(Y, hb + 2 * bytes, bytes);
+        return BNTT_OK;
+    }
     hipError_t e = hipMemcpyAsync(dx, X, bytes, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(dtw, tw, bytes, hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync H2D");

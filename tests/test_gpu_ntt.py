@@ -38,19 +38,23 @@ def test_golden_host_entry(gpu, fixtures, n):
     assert np.array_equal(y, fixtures[f"ver_{n}"])
 
 
-@pytest.mark.parametrize("L", list(range(1, 13)))
+@pytest.mark.parametrize("L", list(range(1, 19)))
 def test_host_entry_sizes(gpu, orc, L):
-    """Host-array calls at every one-workgroup size (the kernel posts a
-    completion word to pinned memory, which the call waits on): consecutive
-    calls with fresh inputs each match the oracle, the last with a value above
-    q (the literal formulas)."""
+    """Host-array calls at every size up to 2^18: zero-copy up to 2^12 (the
+    kernel posts completion words to pinned memory, which the call waits
+    on), staged through pinned memory with the table moved by DMA and x read
+    in place up to 2^16 (completion words for the one-launch split at 2^13,
+    the stream above), and the runtime's copies above that.  Consecutive calls
+    with fresh inputs each match the oracle, the last with a value above q
+    (the literal formulas)."""
     import benchntt_amd as bn
 
     n = 1 << L
     tw = uniform(n, QN, 300 + L)
-    for r in range(6):
+    calls = 6 if L <= 12 else 3
+    for r in range(calls):
         x = uniform(n, QN, 310 + 16 * L + r)
-        if r == 5:
+        if r == calls - 1:
             x[0] = [2**64 - 1, 2**64 - 1]
         y = np.zeros_like(x)
         bn.ntt(y, x, QN, MUN, tw)
@@ -79,6 +83,8 @@ def test_host_entry_table_changes(gpu, orc):
     tw1[700] = uniform(1, QN, 904)[0]
     call(x, tw1)
     call(uniform(4096, QN, 905), uniform(4096, QN, 906))
+    call(uniform(1 << 14, QN, 908), uniform(1 << 14, QN, 909))
+    call(uniform(1 << 17, QN, 910), uniform(1 << 17, QN, 911))
     call(uniform(16, QN, 907), tw1[:16].copy())
     call(x, tw1)
     tw1[1000, 1] |= np.uint64(1 << 62)
