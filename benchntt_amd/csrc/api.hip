@@ -232,6 +232,11 @@ int stage_for_current(Staging **out)
     return BNTT_OK;
 }
 
+// Completion words of the staging: one per workgroup of the widest launch that
+// posts them (the split of a lone transform).
+constexpr size_t kFlagWords = 16;
+static_assert(bntt::lone_parts(bntt::kLoneSplitMaxLog2) <= (int)kFlagWords, "a completion word per workgroup");
+
 // Largest arrays (bytes each) that the host-pointer entry points move zero-copy.
 constexpr size_t kZeroCopyBytes = (size_t)4096 * sizeof(bntt_u128);
 // Host-array transforms up to this size (2^16 points) are staged through
@@ -268,8 +273,9 @@ int stage_flag(Staging &stg)
 {
     if (stg.flag_h) return BNTT_OK;
     void *h = nullptr, *d = nullptr;
-    hipError_t e = hipHostMalloc(&h, 64, hipHostMallocDefault);
-    if (e != hipSuccess) return fail(BNTT_ENOMEM, "hipHostMalloc(64): %s", hipGetErrorString(e));
+    hipError_t e = hipHostMalloc(&h, kFlagWords * sizeof(uint32_t), hipHostMallocDefault);
+    if (e != hipSuccess)
+        return fail(BNTT_ENOMEM, "hipHostMalloc(%zu): %s", kFlagWords * sizeof(uint32_t), hipGetErrorString(e));
     e = hipHostGetDevicePointer(&d, h, 0);
     if (e != hipSuccess) {
         (void)hipHostFree(h);
@@ -277,8 +283,8 @@ int stage_flag(Staging &stg)
     }
     stg.flag_h = static_cast<uint32_t *>(h);
     stg.flag_d = static_cast<uint32_t *>(d);
-    // 16 words: one per workgroup of a launch that posts completion words
-    for (int w = 0; w < 16; w++) __atomic_store_n(stg.flag_h + w, stg.seq, __ATOMIC_RELEASE);
+    // one word per workgroup of a launch that posts completion words
+    for (size_t w = 0; w < kFlagWords; w++) __atomic_store_n(stg.flag_h + w, stg.seq, __ATOMIC_RELEASE);
     return BNTT_OK;
 }
 

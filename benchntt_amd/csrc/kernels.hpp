@@ -47,6 +47,11 @@ constexpr int kLdsMaxLog2 = 12;   // transforms up to 4096 points run in one LDS
 constexpr int kLoneSplitMinLog2 = 11;  // a lone transform from 2^11 points given a tmp splits over workgroups
 constexpr int kLoneSplitMaxLog2 = 13;  // ... up to 2^13 points (beyond, the barrier over 32 workgroups costs more than it saves)
 constexpr int kSpreadMinLog2 = 11;  // ... except a lone one from 2^11 points given a tmp under capture: two launches
+// Workgroups of a lone transform of 2^L points given a tmp (1: one workgroup).
+__host__ __device__ constexpr int lone_parts(int L)
+{
+    return L == 11 ? 4 : L == 12 ? 8 : L == 13 ? 16 : 1;
+}
 constexpr int kMinLog2 = 1;
 constexpr int kMaxLog2 = 24;
 

@@ -164,11 +164,6 @@ struct LoneShape {
     static_assert(LPG1 <= W && LPG2 <= W, "a group's threads must share a wavefront");
 };
 
-// Workgroups of a lone transform of 2^L points given a tmp (1: one workgroup).
-__host__ __device__ constexpr int lone_parts(int L)
-{
-    return L == 11 ? 4 : L == 12 ? 8 : L == 13 ? 16 : 1;
-}
 // Grid-barrier polls before a workgroup gives up on its siblings: each poll is
 // a device-scope load of about 0.5-1 us, so about one to two seconds (every
 // workgroup of this library's kernels is resident far sooner).
