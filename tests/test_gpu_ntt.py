@@ -143,6 +143,20 @@ def test_in_place(gpu, orc, L):
     assert np.array_equal(ntt_dev(gpu, x, tw, inplace=True), orc.ntt(x, tw, QN, MUN))
 
 
+@pytest.mark.parametrize("L", [10, 13, 17])
+def test_host_entry_in_place(gpu, orc, L):
+    """A host-array call with Y the same array as X (the reference's ntt()
+    reads all of X before it writes Y) on each host path: zero-copy (2^10),
+    staged through pinned memory (2^13) and the runtime's copies (2^17)."""
+    import benchntt_amd as bn
+
+    n = 1 << L
+    x, tw = uniform(n, QN, 970 + L), uniform(n, QN, 971 + L)
+    want = orc.ntt(x, tw, QN, MUN)
+    bn.ntt(x, x, QN, MUN, tw)
+    assert np.array_equal(x, want)
+
+
 @pytest.mark.parametrize("L", [6, 10, 12])
 def test_shifted_overlap_rejected(gpu, L):
     """y = x + N (a shifted overlap) is an argument error for one-pass sizes
