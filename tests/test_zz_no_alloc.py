@@ -48,8 +48,10 @@ def noalloc(torch, dx, dtw, L, count):
 def test_noalloc_matches_tmp_entry(gpu):
     """Two calls in a row: the whole array (one 2^20-point transform, two
     passes), then a smaller input (5 transforms of 2^14); then a one-pass
-    size, which uses no scratch."""
-    for L, count, seed in ((20, 1, 1), (14, 5, 2), (10, 7, 3)):
+    size, which uses no scratch; then one transform of 2^13 and of 2^11
+    points, split over several workgroups with their barrier words in the
+    array (twice at 2^13: the second launch meets the first one's words)."""
+    for L, count, seed in ((20, 1, 1), (14, 5, 2), (10, 7, 3), (13, 1, 4), (13, 1, 5), (11, 1, 6)):
         n = 1 << L
         print(f"noalloc L={L} count={count}", flush=True)
         x, tw = uniform(count * n, QN, 100 + seed), uniform(n, QN, 200 + seed)
