@@ -194,11 +194,24 @@ __device__ __forceinline__ void lone_butterfly(U128 &a, U128 &b, const R31 &t, c
 template <int K, bool FAST, bool PM>
 __device__ __forceinline__ void lone_group_stages(U128 &a, U128 &b, const R31 *t, const uint32_t *tw_idx,
                                                   const U128 *__restrict__ tw, U128 *xw, int slot_stride,
-                                                  int slot_off, int j, const Mod &md)
+                                                  int slot_off, int j, const Mod &md, int half = -1)
 {
     constexpr int H = 1 << (K - 1);
 #pragma unroll
     for (int r = 0; r < K; r++) {
+        // half >= 0: the two wavefronts that share a SIMD (half 0 and 1; the
+        // workgroup's eight wavefronts go to the CU's four SIMDs in turn, so
+        // wavefronts w and w + 4 pair up) take turns at the higher issue
+        // priority, one stage each.  They then finish the pass together,
+        // rather than the older one first and the younger one alone, at one
+        // wavefront's slower pace (measured: 6.81 -> 6.70 us per call at 2^10,
+        // 7.29 -> 6.95 us for 64 transforms).
+        if (half >= 0) {
+            if (((r + half) & 1) != 0)
+                __builtin_amdgcn_s_setprio(1);
+            else
+                __builtin_amdgcn_s_setprio(0);
+        }
         lone_butterfly<FAST, PM>(a, b, t[r], tw + tw_idx[r], md);
         if (r + 1 < K) {
             xw[(2 * j) * slot_stride + slot_off] = a;
@@ -261,9 +274,11 @@ __global__ __launch_bounds__((LoneShape<L, P>::T)) void ntt_lone_kernel(
         tr[r] = to_r31(traw[r]);
     }
 
+    // two wavefronts per SIMD (eight in the workgroup): alternate their priority
+    const int half = S::NW == 8 ? (wv >> 2) : -1;
     const bool slow1 = !PM || __any(big1 || above_q(a, md.m) || above_q(b, md.m));
     if (!slow1) {
-        if constexpr (PM) lone_group_stages<K1, true, PM>(a, b, tr, idx, tw, xw, S::GPW1, g1, j1, md);
+        if constexpr (PM) lone_group_stages<K1, true, PM>(a, b, tr, idx, tw, xw, S::GPW1, g1, j1, md, half);
     } else {
         lone_group_stages<K1, false, PM>(a, b, tr, idx, tw, xw, S::GPW1, g1, j1, md);
     }
@@ -305,7 +320,8 @@ __global__ __launch_bounds__((LoneShape<L, P>::T)) void ntt_lone_kernel(
         }
         const bool slow2 = !PM || __any(big2 || above_q(a, md.m) || above_q(b, md.m));
         if (!slow2) {
-            if constexpr (PM) lone_group_stages<K2, true, PM>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md);
+            if constexpr (PM)
+                lone_group_stages<K2, true, PM>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md, half);
         } else {
             lone_group_stages<K2, false, PM>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md);
         }
