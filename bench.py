@@ -344,10 +344,11 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     out["ntt_plain_1024_butterflies_per_s"] = round(5120 / (ms1 * 1e-3), 1)
     # the reference's call shape: host arrays in, host array out, one call at a
     # time (PCIe and synchronisation included; the library's zero-copy path),
-    # timed per call in C as the reference times ntt() (mean of the last 100
-    # of 200 calls), and the same calls from Python (bn.ntt on numpy arrays)
+    # timed per call in C as the reference times ntt() (ntt_scalar.c:671-684:
+    # mean of the last 50 of 100 calls), and the same calls from Python
+    # (bn.ntt on numpy arrays)
     yh = np.empty_like(x_np)
-    out["ntt_plain_1024_host_entry_us"] = round(calltime.ntt_ns_per_call(yh, x_np, tw_np, qn, mun, 200) / 1e3, 2)
+    out["ntt_plain_1024_host_entry_us"] = round(calltime.ntt_ns_per_call(yh, x_np, tw_np, qn, mun, 100) / 1e3, 2)
     if not np.array_equal(yh, y1.cpu().numpy().view(np.uint64).reshape(yh.shape)):
         raise RuntimeError("host-array ntt() at N=1024 differs from the device path")
     for _ in range(20):
