@@ -191,12 +191,22 @@ __device__ __forceinline__ void lone_butterfly(U128 &a, U128 &b, const R31 &t, c
 // them next: value v of the thread's group sits at xw[v * slot_stride +
 // slot_off].  tw_idx[r]: this thread's entry of the caller's table at stage r
 // (read again only on the literal path).
-template <int K, bool FAST, bool PM>
+template <int K, bool FAST, bool PM, bool SWZ = false>
 __device__ __forceinline__ void lone_group_stages(U128 &a, U128 &b, const R31 *t, const uint32_t *tw_idx,
                                                   const U128 *__restrict__ tw, U128 *xw, int slot_stride,
                                                   int slot_off, int j, const Mod &md, int half = -1)
 {
     constexpr int H = 1 << (K - 1);
+    // SWZ (pass 2, whose lanes hold consecutive butterflies of one group):
+    // the writes of outputs 2j and 2j + 1 step by two slots, which puts two
+    // lanes of every 8 (or 16) on one LDS bank group; flipping bit 0 of the
+    // slot by bits 3 and 4 spreads them, and the reads of j and j + H stay
+    // conflict-free (a bijection on the wavefront's slots, the same for the
+    // writes and reads of a stage)
+    auto slot = [&](int v) {
+        const int s = v * slot_stride + slot_off;
+        return SWZ ? s ^ (((s >> 3) ^ (s >> 4)) & 1) : s;
+    };
 #pragma unroll
     for (int r = 0; r < K; r++) {
         // half >= 0: the two wavefronts that share a SIMD (half 0 and 1; the
@@ -214,11 +224,11 @@ __device__ __forceinline__ void lone_group_stages(U128 &a, U128 &b, const R31 *t
         }
         lone_butterfly<FAST, PM>(a, b, t[r], tw + tw_idx[r], md);
         if (r + 1 < K) {
-            xw[(2 * j) * slot_stride + slot_off] = a;
-            xw[(2 * j + 1) * slot_stride + slot_off] = b;
+            xw[slot(2 * j)] = a;
+            xw[slot(2 * j + 1)] = b;
             lds_order();
-            a = xw[j * slot_stride + slot_off];
-            b = xw[(j + H) * slot_stride + slot_off];
+            a = xw[slot(j)];
+            b = xw[slot(j + H)];
             lds_order();
         }
     }
@@ -321,9 +331,9 @@ __global__ __launch_bounds__((LoneShape<L, P>::T)) void ntt_lone_kernel(
         const bool slow2 = !PM || __any(big2 || above_q(a, md.m) || above_q(b, md.m));
         if (!slow2) {
             if constexpr (PM)
-                lone_group_stages<K2, true, PM>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md, half);
+                lone_group_stages<K2, true, PM, true>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md, half);
         } else {
-            lone_group_stages<K2, false, PM>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md);
+            lone_group_stages<K2, false, PM, true>(a, b, tr + K1, idx + K1, tw, xw, 1, g2 << K2, j2, md);
         }
         st16(y + (M2 << K2) + 2 * j2, a);
         st16(y + (M2 << K2) + 2 * j2 + 1, b);
