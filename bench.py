@@ -328,7 +328,9 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     and N in {16..512} with 2^23 coefficients, one shared seeded twiddle table
     per N, as one persistent work-queue execution (graph replay) against one
     launch per transform from native code."""
-    from benchntt_amd import calltime
+    import ctypes
+
+    from benchntt_amd import _lib, calltime
     from benchntt_amd.batch import BatchPlan, TaskList, launch_each, load_tune_table
     from benchntt_amd.residues import uniform
 
@@ -338,10 +340,21 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     tw = torch.from_numpy(tw_np.view(np.int64)).to(dev)
     x1 = torch.from_numpy(x_np.view(np.int64)).to(dev)
     y1 = torch.empty_like(x1)
-    ms1 = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), 200, 20, stream)
+    # back-to-back calls of the C entry point on device arrays (HIP events on
+    # the launch stream), and the same calls through the Python wrapper, whose
+    # own cost per call comes close to the kernel's
+    lib = _lib.load()
+    mq, mm = _lib.U128.of(qn), _lib.U128.of(mun)
+    args = (y1.data_ptr(), x1.data_ptr(), tw.data_ptr(), 10, 1, ctypes.byref(mq), ctypes.byref(mm), None,
+            ctypes.c_void_p(stream.cuda_stream))
+    fwd = lib.bntt_ntt_fwd_dev
+    ms1 = time_kernel(torch, lambda: fwd(*args), 200, 20, stream)
+    _lib.check(fwd(*args))
     out["ntt_plain_1024_us"] = round(ms1 * 1e3, 2)
     out["ntt_plain_1024_ns_per_butterfly"] = round(ms1 * 1e6 / 5120, 4)
     out["ntt_plain_1024_butterflies_per_s"] = round(5120 / (ms1 * 1e-3), 1)
+    msp = time_kernel(torch, lambda: bn.ntt(y1, x1, qn, mun, tw), 200, 20, stream)
+    out["ntt_plain_1024_python_us"] = round(msp * 1e3, 2)
     # the reference's call shape: host arrays in, host array out, one call at a
     # time (PCIe and synchronisation included; the library's zero-copy path),
     # timed per call in C as the reference times ntt() (ntt_scalar.c:671-684:
@@ -357,7 +370,7 @@ def uniform_rows(torch, np, bn, qn, mun, stream, per_launch: bool = True, cpu: b
     for _ in range(200):
         bn.ntt(yh, x_np, qn, mun, tw_np)
     out["ntt_plain_1024_host_entry_python_us"] = round((time.perf_counter() - t0) / 200 * 1e6, 2)
-    log(f"ntt plain 1024: {ms1 * 1e3:.2f} us per launch on device arrays, "
+    log(f"ntt plain 1024: {ms1 * 1e3:.2f} us per launch on device arrays ({msp * 1e3:.2f} from Python), "
         f"{out['ntt_plain_1024_host_entry_us']} us per host-array call timed in C, "
         f"{out['ntt_plain_1024_host_entry_python_us']} us from Python")
     tune = load_tune_table()
