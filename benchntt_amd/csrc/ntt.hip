@@ -118,7 +118,7 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
     }
 }
 
-// Lone transforms of 2^L points, L <= 12: one thread per butterfly of a
+// Lone transforms of 2^L points, L <= 13: one thread per butterfly of a
 // stage, so a stage costs one butterfly's dependency chain rather than a CU's
 // share of the whole network.  A transform of up to 128 points is one group of
 // one pass; larger ones take two passes of K1 and L - K1 stages over groups of
