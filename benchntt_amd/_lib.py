@@ -12,6 +12,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libbntt.so")
 
 BNTT_OK = 0
+BNTT_EINVAL = 1  # bad size, pointer or alignment
+BNTT_EHIP = 2  # HIP runtime error
+BNTT_ENODEV = 3  # no gfx950 device
+BNTT_ENOMEM = 4  # device allocation failed
 
 
 class BnttError(RuntimeError):

@@ -292,6 +292,41 @@ def test_bad_sizes(gpu):
         bn.ntt(x[:3], x[:3], QN, MUN, tw3)
 
 
+def test_c_abi_argument_errors(gpu):
+    """The C entry points on bad arguments: count 0 is a successful no-op
+    (nothing is read, null pointers included); log2n outside [1, 24], null or
+    misaligned pointers and a missing tmp above 4096 points are BNTT_EINVAL
+    with a message, and nothing is launched; the host entry rejects sizes
+    outside [1, 24] the same way."""
+    import ctypes
+
+    from benchntt_amd import _lib
+
+    lib = _lib.load()
+    m, u = _lib.U128.of(QN), _lib.U128.of(MUN)
+    x = gpu.zeros((1 << 13, 2), dtype=gpu.int64, device="cuda")
+    y, tw = gpu.empty_like(x), gpu.zeros_like(x)
+    s = ctypes.c_void_p(gpu.cuda.current_stream().cuda_stream)
+
+    def dev(yp, xp, tp, L, count, tmp=None):
+        return lib.bntt_ntt_fwd_dev(yp, xp, tp, L, count, ctypes.byref(m), ctypes.byref(u), tmp, s)
+
+    assert dev(None, None, None, 10, 0) == 0
+    for L in (0, 25, -3):
+        assert dev(y.data_ptr(), x.data_ptr(), tw.data_ptr(), L, 1) == _lib.BNTT_EINVAL
+        assert "log2n" in lib.bntt_last_error().decode()
+    assert dev(None, x.data_ptr(), tw.data_ptr(), 10, 1) == _lib.BNTT_EINVAL
+    assert dev(y.data_ptr(), x.data_ptr() + 8, tw.data_ptr(), 10, 1) == _lib.BNTT_EINVAL
+    assert "aligned" in lib.bntt_last_error().decode()
+    assert dev(y.data_ptr(), x.data_ptr(), tw.data_ptr(), 13, 1) == _lib.BNTT_EINVAL
+    assert "tmp" in lib.bntt_last_error().decode()
+    xh = np.zeros((16, 2), np.uint64)
+    for L in (0, 25):
+        rc = lib.bntt_ntt_fwd(xh.ctypes.data, xh.ctypes.data, xh.ctypes.data, L, ctypes.byref(m), ctypes.byref(u))
+        assert rc == _lib.BNTT_EINVAL
+    gpu.cuda.synchronize()
+
+
 @pytest.mark.parametrize("L", [11, 12, 13])
 def test_lone_transform_split(gpu, orc, L):
     """A lone transform of 2^11 .. 2^13 points given a tmp (2^L + 16
