@@ -181,7 +181,7 @@ int check_vec(const void *c, const void *a, const void *b, size_t n, const bntt_
 }
 
 int blas_dev(int op, void *c, const void *a, const void *b, const bntt_u128 *x, size_t n, const bntt_u128 *m,
-             const bntt_u128 *mu, void *stream)
+             const bntt_u128 *mu, void *stream, uint32_t *done = nullptr, uint32_t seq = 0)
 {
     int rc = ensure_init();
     if (rc) return rc;
@@ -193,7 +193,8 @@ int blas_dev(int op, void *c, const void *a, const void *b, const bntt_u128 *x, 
     const Mod md = make_mod(m, mu);
     const U128 xs = x ? bntt::make_u128(x->lo, x->hi) : bntt::make_u128(0, 0);
     hipError_t e = bntt::launch_blas(op, static_cast<U128 *>(c), static_cast<const U128 *>(a),
-                                     static_cast<const U128 *>(b), xs, n, md, static_cast<hipStream_t>(stream));
+                                     static_cast<const U128 *>(b), xs, n, md, static_cast<hipStream_t>(stream), done,
+                                     seq);
     if (e != hipSuccess) return hip_fail(e, "blas kernel launch");
     return BNTT_OK;
 }
@@ -233,8 +234,9 @@ int stage_for_current(Staging **out)
 }
 
 // Completion words of the staging: one per workgroup of the widest launch that
-// posts them (the split of a lone transform).
-constexpr size_t kFlagWords = 16;
+// posts them (a BLAS op of 4096 residues, 32 workgroups; the split of a lone
+// transform, 16).
+constexpr size_t kFlagWords = 32;
 static_assert(bntt::lone_parts(bntt::kLoneSplitMaxLog2) <= (int)kFlagWords, "a completion word per workgroup");
 
 // Largest arrays (bytes each) that the host-pointer entry points move zero-copy.
@@ -356,14 +358,19 @@ int blas_host(int op, bntt_u128 *c, const bntt_u128 *a, const bntt_u128 *b, cons
     if (rc) return rc;
     hipStream_t s = stg.stream;
     if (bytes <= kZeroCopyBytes) {
-        if ((rc = stage_host_reserve(stg, 3 * bytes))) return rc;
+        // the kernel reads a and b from pinned memory and writes c there;
+        // each of its workgroups posts a completion word, which the call
+        // waits on instead of the stream
+        if ((rc = stage_host_reserve(stg, 3 * bytes)) || (rc = stage_flag(stg))) return rc;
         char *hb = static_cast<char *>(stg.hbuf), *hd = static_cast<char *>(stg.hdev);
         std::memcpy(hb, a, bytes);
         std::memcpy(hb + bytes, b, bytes);
-        rc = blas_dev(op, hd + 2 * bytes, hd, hd + bytes, x, n, m, mu, s);
+        const size_t words = bntt::blas_blocks(n);
+        if (words > kFlagWords) return fail(BNTT_EHIP, "internal: %zu completion words needed", words);
+        const uint32_t seq = ++stg.seq;
+        rc = blas_dev(op, hd + 2 * bytes, hd, hd + bytes, x, n, m, mu, s, stg.flag_d, seq);
         if (rc) return rc;
-        const hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+        if ((rc = wait_flag(stg, seq, (int)words))) return rc;
         std::memcpy(c, hb + 2 * bytes, bytes);
         return BNTT_OK;
     }

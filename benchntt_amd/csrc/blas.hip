@@ -52,9 +52,12 @@ __device__ __forceinline__ void store(U128 *p, const U128 &a)
     *reinterpret_cast<v4u *>(p) = t;
 }
 
-template <int OP, bool PM>
+// DONE (host-array calls): workgroup w posts seq to done[w] once its outputs
+// are visible system-wide.
+template <int OP, bool PM, bool DONE>
 __global__ __launch_bounds__(kBlasThreads) void blas_kernel(U128 *__restrict__ c, const U128 *__restrict__ a,
-                                                            const U128 *__restrict__ b, U128 x, size_t n, Mod md)
+                                                            const U128 *__restrict__ b, U128 x, size_t n, Mod md,
+                                                            uint32_t *done, uint32_t seq)
 {
     const size_t base = xcd_chunk(blockIdx.x, gridDim.x) * (kBlasThreads * kPerLane) + threadIdx.x;
     U128 av[kPerLane], bv[kPerLane];
@@ -82,36 +85,46 @@ __global__ __launch_bounds__(kBlasThreads) void blas_kernel(U128 *__restrict__ c
             store(c + i, r);
         }
     }
+    if constexpr (DONE) {
+        __threadfence_system();
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_store(done + blockIdx.x, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
+
+size_t blas_blocks(size_t n) { return (n + (size_t)kBlasThreads * kPerLane - 1) / ((size_t)kBlasThreads * kPerLane); }
 
 template <int OP, bool PM>
 static hipError_t launch_op(U128 *c, const U128 *a, const U128 *b, U128 x, size_t n, const Mod &md,
-                            hipStream_t stream)
+                            hipStream_t stream, uint32_t *done, uint32_t seq)
 {
-    const size_t per_block = (size_t)kBlasThreads * kPerLane;
-    const size_t blocks = (n + per_block - 1) / per_block;
+    const size_t blocks = blas_blocks(n);
     if (blocks > 0x7fffffffu) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((blas_kernel<OP, PM>), dim3((unsigned)blocks), dim3(kBlasThreads), 0, stream, c, a, b, x, n,
-                       md);
+    if (done)
+        hipLaunchKernelGGL((blas_kernel<OP, PM, true>), dim3((unsigned)blocks), dim3(kBlasThreads), 0, stream, c, a,
+                           b, x, n, md, done, seq);
+    else
+        hipLaunchKernelGGL((blas_kernel<OP, PM, false>), dim3((unsigned)blocks), dim3(kBlasThreads), 0, stream, c, a,
+                           b, x, n, md, nullptr, 0u);
     return hipGetLastError();
 }
 
 hipError_t launch_blas(int op, U128 *c, const U128 *a, const U128 *b, U128 x, size_t n, const Mod &md,
-                       hipStream_t stream)
+                       hipStream_t stream, uint32_t *done, uint32_t seq)
 {
     if (n == 0) return hipSuccess;
     const bool pm = md.pm != 0;
     switch (op) {
     case OP_VVADD:
-        return launch_op<OP_VVADD, false>(c, a, b, x, n, md, stream);
+        return launch_op<OP_VVADD, false>(c, a, b, x, n, md, stream, done, seq);
     case OP_VVSUB:
-        return launch_op<OP_VVSUB, false>(c, a, b, x, n, md, stream);
+        return launch_op<OP_VVSUB, false>(c, a, b, x, n, md, stream, done, seq);
     case OP_VVMUL:
-        return pm ? launch_op<OP_VVMUL, true>(c, a, b, x, n, md, stream)
-                  : launch_op<OP_VVMUL, false>(c, a, b, x, n, md, stream);
+        return pm ? launch_op<OP_VVMUL, true>(c, a, b, x, n, md, stream, done, seq)
+                  : launch_op<OP_VVMUL, false>(c, a, b, x, n, md, stream, done, seq);
     case OP_AXPY:
-        return pm ? launch_op<OP_AXPY, true>(c, a, b, x, n, md, stream)
-                  : launch_op<OP_AXPY, false>(c, a, b, x, n, md, stream);
+        return pm ? launch_op<OP_AXPY, true>(c, a, b, x, n, md, stream, done, seq)
+                  : launch_op<OP_AXPY, false>(c, a, b, x, n, md, stream, done, seq);
     default:
         return hipErrorInvalidValue;
     }

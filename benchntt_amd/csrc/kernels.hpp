@@ -26,8 +26,11 @@ __device__ __forceinline__ size_t xcd_chunk(size_t blk, size_t nblocks)
 }
 
 // Elementwise ops over n residues (device pointers).  x is the axpy scalar.
+// With `done` (host-array calls) each of the launch's blas_blocks(n)
+// workgroups w posts seq to done[w] once its outputs are visible system-wide.
 hipError_t launch_blas(int op, U128 *c, const U128 *a, const U128 *b, U128 x, size_t n, const Mod &md,
-                       hipStream_t stream);
+                       hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0);
+size_t blas_blocks(size_t n);
 
 // Forward constant-geometry NTT of `count` transforms of size 2^log2n, stored
 // contiguously, all with twiddle table tw (2^log2n entries).  tmp must hold
