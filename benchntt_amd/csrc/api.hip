@@ -559,7 +559,8 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
         const hipError_t e =
             bntt::launch_ntt_uniform(reinterpret_cast<U128 *>(hd + 2 * bytes), reinterpret_cast<const U128 *>(hd),
                                      reinterpret_cast<const U128 *>(hd + bytes), log2n, 1, make_mod(m, mu),
-                                     tmpb ? static_cast<U128 *>(stg.buf) : nullptr, s, stg.flag_d, seq, &words);
+                                     tmpb ? static_cast<U128 *>(stg.buf) : nullptr, s, stg.flag_d, seq, &words,
+                                     (int)kFlagWords);
         if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
         if ((rc = wait_flag(stg, seq, words))) return rc;
         std::memcpy(Y, hb + 2 * bytes, bytes);
@@ -587,10 +588,14 @@ int bntt_ntt_fwd(bntt_u128 *Y, const bntt_u128 *X, const bntt_u128 *tw, int log2
             const uint32_t seq = ++stg.seq;
             int words = 1;
             e = bntt::launch_ntt_uniform(yh, static_cast<const U128 *>(dx), static_cast<const U128 *>(dtw), log2n, 1,
-                                         make_mod(m, mu), static_cast<U128 *>(dtmp), s, stg.flag_d, seq, &words);
+                                         make_mod(m, mu), static_cast<U128 *>(dtmp), s, stg.flag_d, seq, &words,
+                                         (int)kFlagWords);
             if (e != hipSuccess) return hip_fail(e, "ntt kernel launch");
             if ((rc = wait_flag(stg, seq, words))) return rc;
         } else {
+            // several passes: completion words from the last pass's 32..128
+            // workgroups, each releasing at system scope, measured slower
+            // than the stream (58.9 against 53.4 us at 2^14)
             if ((rc = bntt_ntt_fwd_dev(yh, dx, dtw, log2n, 1, m, mu, dtmp, s))) return rc;
             e = hipStreamSynchronize(s);
             if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");

@@ -39,12 +39,14 @@ size_t blas_blocks(size_t n);
 // hold 2^log2n + 16 residues (the intermediate and the grid-barrier words),
 // and the transform splits over several workgroups (or, in a launch being
 // captured into a graph, takes one workgroup or two launches).  x and y may
-// alias only when log2n <= kLdsMaxLog2.  With `done` (one transform of at
-// most 2^kLdsMaxLog2 points) each of the launch's *done_words workgroups w
-// stores `seq` to done[w], at system scope, after its outputs.
+// alias only when log2n <= kLdsMaxLog2.  With `done` (one transform of at most
+// 2^kLoneSplitMaxLog2 points, one launch) each of the launch's *done_words
+// workgroups w stores `seq` to done[w], at system scope, after its outputs; a
+// launch that would need more than done_cap words is refused
+// (hipErrorInvalidValue) before anything runs.
 hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int log2n, size_t count, const Mod &md,
                               U128 *tmp, hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0,
-                              int *done_words = nullptr);
+                              int *done_words = nullptr, int done_cap = 0);
 
 constexpr int kLdsMaxLog2 = 12;   // transforms up to 4096 points run in one LDS-resident pass
 constexpr int kLoneSplitMinLog2 = 11;  // a lone transform from 2^11 points given a tmp splits over workgroups

@@ -109,12 +109,13 @@ __global__ __launch_bounds__(kMaxThreads) void ntt_pass_kernel(U128 *__restrict_
         if ((e >> k) < nvalid) st16(yb + e, lds[lidx(e, k)]);
     }
     if (done) {
-        // a one-workgroup launch whose completion the host reads from pinned
-        // memory: every wave makes its stores visible system-wide, then one
-        // release store posts the sequence number
+        // a launch whose completion the host reads from pinned memory: every
+        // wave makes its stores visible system-wide, then one release store
+        // posts the sequence number to this workgroup's word
         __threadfence_system();
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_store(done, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        if (threadIdx.x == 0)
+            __hip_atomic_store(done + blockIdx.x, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     }
 }
 
@@ -405,7 +406,8 @@ static hipError_t launch_lone(U128 *y, const U128 *x, const U128 *tw, int L, siz
 
 template <int ELOG, bool PM>
 static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int S, int k, int Glog, size_t count,
-                              const Mod &md, hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0)
+                              const Mod &md, hipStream_t stream, uint32_t *done = nullptr, uint32_t seq = 0,
+                              int *nblocks = nullptr, int done_cap = 0)
 {
     const size_t total_groups = count << (L - k);
     const size_t blocks = (total_groups + ((size_t)1 << Glog) - 1) >> Glog;
@@ -418,7 +420,8 @@ static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int
     const size_t lds_bytes = pass_lds_bytes(k, S, Glog, klds);
     if (threads < 1 || threads > kMaxThreads || blocks > 0x7fffffffu || lds_bytes > kMaxLdsBytes)
         return hipErrorInvalidConfiguration;
-    if (done && blocks != 1) return hipErrorInvalidValue;
+    if (done && blocks > (size_t)done_cap) return hipErrorInvalidValue;  // a word per workgroup
+    if (nblocks) *nblocks = (int)blocks;
     // raise the dynamic LDS limit once per instantiation and device
     static std::atomic<uint64_t> lds_done{0};
     const hipError_t attr =
@@ -432,15 +435,15 @@ static hipError_t launch_pass(U128 *y, const U128 *x, const U128 *tw, int L, int
 template <bool PM>
 static hipError_t launch_pass_e(U128 *y, const U128 *x, const U128 *tw, int L, int S, int k, int Glog, int elog,
                                 size_t count, const Mod &md, hipStream_t stream, uint32_t *done = nullptr,
-                                uint32_t seq = 0)
+                                uint32_t seq = 0, int *nblocks = nullptr, int done_cap = 0)
 {
     switch (elog) {
     case 1:
-        return launch_pass<1, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq);
+        return launch_pass<1, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq, nblocks, done_cap);
     case 2:
-        return launch_pass<2, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq);
+        return launch_pass<2, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq, nblocks, done_cap);
     default:
-        return launch_pass<3, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq);
+        return launch_pass<3, PM>(y, x, tw, L, S, k, Glog, count, md, stream, done, seq, nblocks, done_cap);
     }
 }
 
@@ -453,14 +456,15 @@ constexpr size_t kFewTransforms = 64;
 constexpr int kLoneMaxLog2 = 10;
 
 hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, size_t count, const Mod &md, U128 *tmp,
-                              hipStream_t stream, uint32_t *done, uint32_t seq, int *done_words)
+                              hipStream_t stream, uint32_t *done, uint32_t seq, int *done_words, int done_cap)
 {
     if (done_words) *done_words = 1;
     if (count == 0) return hipSuccess;
     if (L < kMinLog2 || L > kMaxLog2) return hipErrorInvalidValue;
-    // a completion flag needs one transform of at most 2^13 points, and above
-    // 4096 points the split launch
-    if (done && (count != 1 || L > kLoneSplitMaxLog2)) return hipErrorInvalidValue;
+    // completion words: one transform of at most 2^13 points in one launch,
+    // each of whose workgroups posts one (*done_words of them, at most
+    // done_cap)
+    if (done && (count != 1 || L > kLoneSplitMaxLog2 || done_cap < 1)) return hipErrorInvalidValue;
     const bool pm = md.pm != 0;
     // A lone transform of 2^11 .. 2^13 points given a tmp splits over several
     // workgroups joined by a grid barrier.  Its per-launch token would repeat
@@ -471,12 +475,12 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
         const hipError_t ce = hipStreamIsCapturing(stream, &cs);
         if (ce != hipSuccess) return ce;
         if (cs == hipStreamCaptureStatusNone) {
+            if (done && lone_parts(L) > done_cap) return hipErrorInvalidValue;
             if (done_words) *done_words = lone_parts(L);
             return pm ? launch_lone<true>(y, x, tw, L, 1, md, tmp, stream, done, seq)
                       : launch_lone<false>(y, x, tw, L, 1, md, tmp, stream, done, seq);
         }
     }
-    if (done && L > kLdsMaxLog2) return hipErrorInvalidValue;
     if (L <= kLoneMaxLog2 && count <= kFewTransforms)
         return pm ? launch_lone<true>(y, x, tw, L, count, md, nullptr, stream, done, seq)
                   : launch_lone<false>(y, x, tw, L, count, md, nullptr, stream, done, seq);
@@ -496,8 +500,10 @@ hipError_t launch_ntt_uniform(U128 *y, const U128 *x, const U128 *tw, int L, siz
             Glog = 8 + elog - L;
             if (Glog < 0) Glog = 0;
         }
-        return pm ? launch_pass_e<true>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq)
-                  : launch_pass_e<false>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq);
+        return pm ? launch_pass_e<true>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq, done_words,
+                                        done_cap)
+                  : launch_pass_e<false>(y, x, tw, L, 0, L, Glog, elog, count, md, stream, done, seq, done_words,
+                                         done_cap);
     }
     if (tmp == nullptr) return hipErrorInvalidValue;
     const int np = L <= 16 ? 2 : 3;
