@@ -101,9 +101,25 @@ def host_info() -> dict:
     except Exception:
         pass
     fl = cpu_flags()
-    return {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "threads_used": cpu_threads(),
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)), "threads_used": cpu_threads(),
             "model": model, "isa": sorted(f for f in fl if f.startswith(("avx", "amx")) or f in ZNVER4_FLAGS),
             "flavour": flavour(), "compiler_flags": FLAVOUR_FLAGS[flavour()]}
+    # a virtualised host may report only the brand as its model name: the
+    # family / model / stepping numbers and the clock identify the generation
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                key, _, val = line.partition(":")
+                key, val = key.strip(), val.strip()
+                names = {"cpu family": "cpu_family", "model": "cpu_model_number", "stepping": "cpu_stepping",
+                         "cpu MHz": "cpu_mhz", "cache size": "cpu_cache"}
+                if key in names and names[key] not in info:
+                    info[names[key]] = val
+                if not line.strip():
+                    break
+    except OSError:
+        pass
+    return info
 
 
 def cpu_model() -> str:
