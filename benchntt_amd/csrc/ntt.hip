@@ -213,12 +213,13 @@ __device__ __forceinline__ void lone_group_stages(U128 &a, U128 &b, const R31 *t
         // half >= 0: the two wavefronts that share a SIMD (half 0 and 1; the
         // workgroup's eight wavefronts go to the CU's four SIMDs in turn, so
         // wavefronts w and w + 4 pair up) take turns at the higher issue
-        // priority, one stage each.  They then finish the pass together,
-        // rather than the older one first and the younger one alone, at one
-        // wavefront's slower pace (measured: 6.81 -> 6.70 us per call at 2^10,
-        // 7.29 -> 6.95 us for 64 transforms).
+        // priority, one stage each, the older one first.  They then finish the
+        // pass together, rather than the older one first and the younger one
+        // alone, at one wavefront's slower pace (measured: 6.81 -> 6.70 us per
+        // call at 2^10, 7.29 -> 6.95 us for 64 transforms; the older one first
+        // rather than the younger, 6.63 -> 6.55 and 6.90 -> 6.82 us).
         if (half >= 0) {
-            if (((r + half) & 1) != 0)
+            if (((r + half + 1) & 1) != 0)
                 __builtin_amdgcn_s_setprio(1);
             else
                 __builtin_amdgcn_s_setprio(0);
